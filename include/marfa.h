@@ -1,0 +1,141 @@
+/* marfa.h -- C ABI of the MI355X line-by-line (LBL) absorption core.
+ *
+ * Plain C types only (pointers and sizes; no torch or HIP types), so that the Fortran
+ * host binds it with ISO_C_BINDING (marfa_amd/fortran/marfa_capi.f90) and Python with
+ * ctypes (marfa_amd/lbl.py).  Every function returns 0 on success or a positive error
+ * code; none of them calls exit().  Error codes 1-9 keep the meaning of the reference's
+ * `stop` codes where one exists (9 = line-data/window error, src/LBL.f90:50-53); codes
+ * from 20 on are this library's own (see MARFA_E_* below).
+ *
+ * Host buffers are owned by the caller and copied in; device buffers are owned by the
+ * context.  A context is thread-compatible: calls on one context must not overlap.
+ *
+ * Reference interfaces replaced (Razumovskyy/MARFA):
+ *   marfa_lbl_scheme    <- subroutine lblScheme(lineIdxParameter, capWVParameter)
+ *                          (src/LBL.f90:25-100), bracketed by resetAbsorptionGridValues
+ *                          and cascadeInterpolation (src/Grids.f90:64-139) exactly as the
+ *                          only caller does (app/main.f90:148-160)
+ *   marfa_set_lines     <- the 36-byte direct-access line records read at src/LBL.f90:43
+ *                          (layout app/processParFile.f90:176-177)
+ *   marfa_set_tips      <- TIPS(74,501) filled by readTIPS (src/EnvironmentSetup.f90:230-245)
+ *   job.shape_id        <- shapeFuncPtr / setShapeFunction (src/Shapes.f90:8-15), the
+ *                          `shape` plugin interface (src/Interfaces.f90:6-12)
+ *   job.chi_id          <- chiFactorFuncPtr / setChiFactorFunction (src/ChiFactors.f90:7-16),
+ *                          the `chifactor` plugin interface (src/Interfaces.f90:15-23)
+ *   marfa_pt_tables     <- ATMOSPHERIC_LEVELS_LOOP x SUBINTERVALS_LOOP (app/main.f90:81-180)
+ *                          producing every level's PT-table records at once
+ *   marfa_windows       <- determineStartingSpectralLine (app/main.f90:196-248) and the
+ *                          lineIdx / capWV carry (app/main.f90:133-144, src/LBL.f90:57-61)
+ */
+#ifndef MARFA_H
+#define MARFA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MARFA_NT 20481           /* points per PT-table record, src/Grids.f90:16 */
+#define MARFA_RECORD_BYTES 81924 /* NT * 4, app/main.f90:111 */
+
+/* precision forms */
+#define MARFA_FP64 0  /* plain drop-in: the reference's algorithm in FP64 */
+#define MARFA_FP32K 1 /* FP32 profiles + Kahan-compensated FP32 sums, FP64 offsets */
+
+/* shapes (src/Shapes.f90) and chi factors (src/ChiFactors.f90) */
+#define MARFA_SHAPE_VOIGT 0    /* voigt, Shapes.f90:22 */
+#define MARFA_SHAPE_LORENTZ 1  /* lorentzShape (chi-corrected Lorentz), Shapes.f90:135 */
+#define MARFA_CHI_NONE 0
+#define MARFA_CHI_TONKOV 1
+#define MARFA_CHI_POLLACK 2
+#define MARFA_CHI_PERRIN 3
+
+/* targets, app/main.f90:163-168 */
+#define MARFA_ACS 0
+#define MARFA_VAC 1
+
+/* error codes */
+#define MARFA_OK 0
+#define MARFA_E_WINDOW 9      /* line index outside the line list (src/LBL.f90:50-53) */
+#define MARFA_E_ARG 20        /* invalid argument */
+#define MARFA_E_STATE 21      /* lines / TIPS / levels not set */
+#define MARFA_E_HIP 22        /* HIP runtime error (message: marfa_last_error) */
+#define MARFA_E_NODEVICE 23   /* no HIP device */
+#define MARFA_E_TEMPERATURE 24 /* T outside the TIPS table (src/Spectroscopy.f90:110-114) */
+
+typedef struct marfa_ctx marfa_ctx;
+
+typedef struct {
+    double start_wv;  /* startWV [cm-1] */
+    double end_wv;    /* endWV [cm-1] */
+    float cutoff;     /* cutOff [cm-1] */
+    int32_t shape_id; /* MARFA_SHAPE_* */
+    int32_t chi_id;   /* MARFA_CHI_* */
+    int32_t mol_code; /* moleculeIntCode (1 H2O, 2 CO2, ...), getSpeciesCode */
+    int32_t target;   /* MARFA_ACS or MARFA_VAC */
+    int32_t precision;/* MARFA_FP64 or MARFA_FP32K */
+} marfa_job;
+
+/* ---- context ---- */
+int marfa_ctx_create(int device, marfa_ctx **out);
+int marfa_ctx_destroy(marfa_ctx *ctx);
+const char *marfa_last_error(const marfa_ctx *ctx);
+int marfa_version(void);
+
+/* ---- inputs (copied to device) ---- */
+int marfa_set_tips(marfa_ctx *ctx, int32_t niso, int32_t ntemp, const float *tips /* [niso][ntemp] */);
+int marfa_set_lines(marfa_ctx *ctx, int64_t n, const double *nu, const float *S,
+                    const float *gair, const float *gself, const float *elow, const float *nT,
+                    const float *delta, const int32_t *molIso);
+/* levels: total pressure [atm], temperature [K], number density [1/(cm^2 km)] */
+int marfa_set_levels(marfa_ctx *ctx, int32_t nlev, const float *P, const float *T,
+                     const float *rho);
+
+/* ---- windows (host-side index logic) ----
+ * Number of 10 cm-1 subintervals of the job; lo/hi (0-based line ranges [lo,hi)),
+ * recnum (1-based PT-table record numbers) and origin (subinterval starts) are filled
+ * when non-NULL and max_rec is large enough. */
+int marfa_windows(marfa_ctx *ctx, const marfa_job *job, int32_t max_rec, int32_t *nrec,
+                  int64_t *lo, int64_t *hi, int32_t *recnum, double *origin);
+
+/* ---- drop-in for one call of lblScheme between reset and cascade ----
+ * line_idx: 1-based record index of the first line to read (in/out, carried to the
+ * next subinterval as src/LBL.f90:61 does); cap_wv, start_dwv and ext_end_dwv are
+ * capWV, startDeltaWV and extEndDeltaWV; P, T, pSelf the level state.  rk receives the
+ * 20,481-point record (ACS).  Runs synchronously on the context's stream. */
+int marfa_lbl_scheme(marfa_ctx *ctx, int64_t *line_idx, double cap_wv, double start_dwv,
+                     double ext_end_dwv, float cutoff, float P, float T, float pSelf,
+                     int32_t shape_id, int32_t chi_id, int32_t mol_code, int32_t precision,
+                     float *rk);
+
+/* ---- all levels x subintervals of one job ----
+ * Records of level l, subinterval s land at out[(l*nrec + s)*MARFA_NT].
+ * out_dev: device pointer (may be NULL) -- results stay in HBM;
+ * out_host: host pointer (may be NULL) -- results copied back.
+ * stream: hipStream_t to run on (NULL = the context's stream).  Synchronous unless
+ * out_host is NULL and async != 0. */
+int marfa_pt_tables(marfa_ctx *ctx, const marfa_job *job, float *out_dev, float *out_host,
+                    void *stream, int32_t async);
+
+/* exact count of distinct profile evaluations (samples) of the job's grid geometry,
+ * summed over levels and subintervals (computed on the device). */
+int marfa_count_samples(marfa_ctx *ctx, const marfa_job *job, int64_t *samples);
+
+/* Pre-cascade grid sums of one record (level, subinterval index), in the reference's
+ * layout: for grid k = 0..9 the arrays RKkP, RKk, RKkL (src/Grids.f90:34-45) of 10*2^k
+ * cells each, then RK(1..20481) holding the direct fine-point deposits (30,690 + 20,481
+ * doubles); record (optional) receives the finished record.  For tests of the grid
+ * selection against a CPU restatement. */
+int marfa_grid_sums(marfa_ctx *ctx, const marfa_job *job, int32_t level, int32_t rec,
+                    double *grids, float *record);
+
+/* average duration [ms] of the last marfa_pt_tables call's kernels, measured with HIP
+ * events on the stream they ran on: [0] spectroscopy, [1] sort, [2] coarse deposit,
+ * [3] fine deposit + cascade, [4] total. */
+int marfa_last_timings(const marfa_ctx *ctx, float *ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MARFA_H */

@@ -1,0 +1,265 @@
+// lbl_device.h -- device-side constants, line shapes and grid geometry of the LBL core.
+//
+// Line shapes follow the reference's Humlicek-Kuntz Voigt (src/Shapes.f90:22-133), the
+// chi-corrected Lorentz and wing asymptotics (src/Spectroscopy.f90:194-261) and the chi
+// factors (src/ChiFactors.f90:18-122).  Constants keep the FP32 values the reference
+// folds (src/Constants.f90:7-22): their bit patterns are below.
+//
+// Two precision forms share this code:
+//   double  (MARFA_FP64)  every operation in FP64;
+//   float   (MARFA_FP32K) samples in FP32.  The region of the Humlicek-Kuntz split is
+//           decided on the fast path only when the FP32 estimate of VX is clearly in the
+//           Lorentz region; every other sample decides its region from the FP64 value of
+//           VX = |sqrt(ln2) X / gD| formed as the FP64 form forms it, so both forms pick
+//           the same approximation for the same sample.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace marfa {
+
+constexpr int NT = 20481;                 // src/Grids.f90:16
+constexpr double HF = 0.00048828125;      // finest step H = 2^-11
+constexpr double H9 = 0.001953125;        // 2^-9
+
+__host__ __device__ inline float bits_f(uint32_t b) { return __builtin_bit_cast(float, b); }
+
+// bit patterns of the reference's FP32 constants
+#define MARFA_PI_F bits_f(0x40490FDAU)      // PI = 3.1415926
+#define MARFA_SQLN2_F bits_f(0x3F55224DU)   // sqrt(log(2.))
+#define MARFA_C2_F bits_f(0x3FB829CCU)      // h c / k
+#define MARFA_DOPC_F bits_f(0x34C04328U)    // dopplerCONST
+#define MARFA_SQRTPI_F bits_f(0x3FE2DFC4U)  // sqrt(pi)
+#define MARFA_SQRT14_F bits_f(0x3F97739FU)  // sqrt(1.4)
+#define MARFA_LOG2_F bits_f(0x3F317218U)    // log(2.)
+#define MARFA_BOLSI_F 1.3806503e-23f        // src/Constants.f90:14
+#define MARFA_PATM_F 101325.f               // src/Constants.f90:16
+#define MARFA_TREF_F 296.f                  // src/Constants.f90:24
+
+__host__ __device__ inline double hk(int k) { return __builtin_ldexp(1.0, -k); }  // H_k = 2^-k
+
+// ---------------------------------------------------------------- chi factors
+template <typename R> __device__ inline R dexp(R x);
+template <> __device__ inline double dexp<double>(double x) { return exp(x); }
+template <> __device__ inline float dexp<float>(float x) { return __expf(x); }
+
+// chi(|X|), src/ChiFactors.f90:29-122; applied only to CO2 (moleculeIntCode 2)
+template <typename R>
+__device__ inline R chi_factor(int chi, R ax, R B1, R B2) {
+    if (chi == 1) {
+        if (ax > R(3)) {
+            if (ax <= R(150)) return R(1.084f) * dexp<R>(-(R(0.027f) * ax));
+            if (ax <= R(300)) return R(0.208f) * dexp<R>(-(R(0.016f) * ax));
+            return R(0.025f) * dexp<R>(-(R(0.009f) * ax));
+        }
+    } else if (chi == 2) {
+        if (ax >= R(3)) {
+            if (ax < R(10)) return R(1.35f) * dexp<R>(-ax / R(10));
+            return R(0.614f) * dexp<R>(-ax / R(47));
+        }
+    } else if (chi == 3) {
+        const R B3 = R(0.0232f);
+        if (ax > R(3)) {
+            if (ax < R(30)) return dexp<R>(-B1 * (ax - R(3)));
+            if (ax < R(120)) return dexp<R>(-B1 * R(27) - B2 * (ax - R(30)));
+            return dexp<R>(-B1 * R(27) - B2 * R(90) - B3 * (ax - R(120)));
+        }
+    }
+    return R(1);
+}
+
+// ---------------------------------------------------------------- per-line state
+// Derived once per (line, level) visit; R is the sample precision.
+template <typename R>
+struct LineShape {
+    R xs;     // sqrt(ln2)/gD    : VX = |X| xs
+    R vy;     // sqrt(ln2) gL/gD : VY
+    R lorc;   // gL S / PI       : Lorentz numerator
+    R gl2;    // gL^2
+    R normc;  // S / sqrt(pi) / (gD/sqrt(ln2)) : K(x,y) -> profile
+    R dopa;   // sqrt(ln2)/(sqrt(pi) gD) S    : Doppler amplitude
+    R igd;    // 1/gD
+    R gL, S;
+    double gD64, gL64;  // region decisions (FP64, as the FP64 form forms them)
+};
+
+template <typename R>
+__device__ inline LineShape<R> make_line(double gD, double gL, double S) {
+    LineShape<R> p;
+    const double sq = (double)MARFA_SQLN2_F;
+    if constexpr (sizeof(R) == 8) {
+        p.xs = sq / gD;
+        p.vy = sq * gL / gD;
+        p.lorc = gL * S / (double)MARFA_PI_F;
+        p.gl2 = gL * gL;
+        p.normc = S / (double)MARFA_SQRTPI_F / (gD / sq);
+        p.dopa = sq / ((double)MARFA_SQRTPI_F * gD) * S;
+        p.igd = 1.0 / gD;
+        p.gL = gL;
+        p.S = S;
+    } else {
+        const float g = (float)gD, l = (float)gL, s = (float)S;
+        p.xs = MARFA_SQLN2_F / g;
+        p.vy = (float)(sq * gL / gD);
+        p.lorc = l * s / MARFA_PI_F;
+        p.gl2 = l * l;
+        p.normc = s / MARFA_SQRTPI_F / (g / MARFA_SQLN2_F);
+        p.dopa = MARFA_SQLN2_F / (MARFA_SQRTPI_F * g) * s;
+        p.igd = 1.0f / g;
+        p.gL = l;
+        p.S = s;
+    }
+    p.gD64 = gD;
+    p.gL64 = gL;
+    return p;
+}
+
+struct ShapeCtl {
+    int shape_id, chi_id, mol;
+    float B1, B2;     // Perrin temperature coefficients of the level (FP32 as the reference)
+    double B1d, B2d;  // same in FP64
+};
+
+template <typename R>
+__device__ inline R chi_of(const ShapeCtl &c, R ax) {
+    if (c.mol != 2 || c.chi_id == 0) return R(1);
+    if constexpr (sizeof(R) == 8) return chi_factor<double>(c.chi_id, ax, c.B1d, c.B2d);
+    else return chi_factor<float>(c.chi_id, ax, c.B1, c.B2);
+}
+
+// Humlicek-Kuntz K(x,y) rational regions 1-3 (src/Shapes.f90:60-117), VY = y, V2 = x^2
+template <typename R>
+__device__ inline R kuntz_rational(int region, R VY, R V2) {
+    R num, den;
+    if (region == 1) {
+        R Y2 = VY * VY;
+        R A1 = (R(0.2820948f) + R(0.5641896f) * Y2) * VY;
+        R B1 = R(0.5641896f) * VY;
+        R A2 = R(0.25f) + Y2 + Y2 * Y2;
+        R B2 = Y2 + Y2 - R(1);
+        num = A1 + B1 * V2;
+        den = A2 + B2 * V2 + V2 * V2;
+    } else if (region == 2) {
+        R Y2 = VY * VY;
+        R A3 = VY * (((R(0.56419f) * Y2 + R(3.10304f)) * Y2 + R(4.65456f)) * Y2 + R(1.05786f));
+        R B3 = VY * ((R(1.69257f) * Y2 + R(0.56419f)) * Y2 + R(2.962f));
+        R C3 = VY * (R(1.69257f) * Y2 - R(2.53885f));
+        R D3 = VY * R(0.56419f);
+        R A4 = (((Y2 + R(6)) * Y2 + R(10.5f)) * Y2 + R(4.5f)) * Y2 + R(0.5625f);
+        R B4 = ((R(4) * Y2 + R(6)) * Y2 + R(9)) * Y2 - R(4.5f);
+        R C4 = R(10.5f) + R(6) * (Y2 - R(1)) * Y2;
+        R D4 = R(4) * Y2 - R(6);
+        num = ((D3 * V2 + C3) * V2 + B3) * V2 + A3;
+        den = (((V2 + D4) * V2 + C4) * V2 + B4) * V2 + A4;
+    } else {
+        R Y = VY;
+        R A5 = ((((((((R(0.564224f) * Y + R(7.55895f)) * Y + R(49.5213f)) * Y + R(204.510f)) * Y +
+                     R(581.746f)) * Y + R(1174.8f)) * Y + R(1678.33f)) * Y + R(1629.76f)) * Y +
+                 R(973.778f)) * Y + R(272.102f);
+        R B5 = ((((((R(2.25689f) * Y + R(22.6778f)) * Y + R(100.705f)) * Y + R(247.198f)) * Y +
+                   R(336.364f)) * Y + R(220.843f)) * Y - R(2.34403f)) * Y - R(60.5644f);
+        R C5 = ((((R(3.38534f) * Y + R(22.6798f)) * Y + R(52.8454f)) * Y + R(42.5683f)) * Y +
+                 R(18.546f)) * Y + R(4.58029f);
+        R D5 = ((R(2.25689f) * Y + R(7.56186f)) * Y + R(1.66203f)) * Y - R(0.128922f);
+        R E5 = R(0.971457E-3f) + R(0.564224f) * Y;
+        R A6 = (((((((((Y + R(13.3988f)) * Y + R(88.2674f)) * Y + R(369.199f)) * Y + R(1074.41f)) * Y +
+                     R(2256.98f)) * Y + R(3447.63f)) * Y + R(3764.97f)) * Y + R(2802.87f)) * Y +
+                 R(1280.83f)) * Y + R(272.102f);
+        R B6 = (((((((R(5) * Y + R(53.5952f)) * Y + R(266.299f)) * Y + R(793.427f)) * Y + R(1549.68f)) * Y +
+                   R(2037.31f)) * Y + R(1758.34f)) * Y + R(902.306f)) * Y + R(211.678f);
+        R C6 = (((((R(10) * Y + R(80.3928f)) * Y + R(269.292f)) * Y + R(479.258f)) * Y + R(497.302f)) * Y +
+                 R(308.186f)) * Y + R(78.866f);
+        R D6 = (((R(10) * Y + R(53.5952f)) * Y + R(92.7586f)) * Y + R(55.0293f)) * Y + R(22.0353f);
+        R E6 = (R(5) * Y + R(13.3988f)) * Y + R(1.49645f);
+        num = (((E5 * V2 + D5) * V2 + C5) * V2 + B5) * V2 + A5;
+        den = ((((V2 + E6) * V2 + D6) * V2 + C6) * V2 + B6) * V2 + A6;
+    }
+    return num / den;
+}
+
+__constant__ static const float ASYM_U[9] = {1.f, 1.5f, 2.f, 2.5f, 3.f, 3.5f, 4.f, 4.5f, 5.f};
+__constant__ static const float ASYM_W[9] = {-0.688f, 0.2667f, 0.6338f, 0.4405f, 0.2529f,
+                                             0.1601f, 0.1131f, 0.0853f, 0.068f};
+
+// Lorentz x chi (src/Spectroscopy.f90:202,227)
+template <typename R>
+__device__ inline R chilor(R X, const LineShape<R> &p, const ShapeCtl &c) {
+    R lor = p.lorc / (X * X + p.gl2);
+    if (c.chi_id != 0 && c.mol == 2) lor *= chi_of<R>(c, X < R(0) ? -X : X);
+    return lor;
+}
+
+template <typename R>
+__device__ inline R doppler(R X, const LineShape<R> &p) {
+    R q = X * p.igd;
+    return p.dopa * dexp<R>(-((q * q) * R(MARFA_LOG2_F)));
+}
+
+// Non-Lorentz part of voigt(X): the region is decided from FP64 VX/VY formed exactly as
+// the FP64 form forms them (src/Shapes.f90:49-50), the value in precision R.
+template <typename R>
+__device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const ShapeCtl &c) {
+    const double sq = (double)MARFA_SQLN2_F;
+    const double VX = fabs((sq * X64) / p.gD64);
+    const double VY = sq * p.gL64 / p.gD64;
+    const R X = (R)X64;
+    if (VX >= 15.0) return chilor<R>(X, p, c);
+    int region;
+    if (VX + VY >= 15.0) region = 1;
+    else if (VX + VY >= 5.5) region = 2;
+    else if (VX <= 1.0 || VY >= (double)0.02f) region = 3;
+    else region = 4;
+    if (region < 4) {
+        R vx = (R)VX;
+        return kuntz_rational<R>(region, (R)VY, vx * vx) * p.normc;
+    }
+    if (VX > 5.0) {
+        R vx = (R)VX;
+        return chilor<R>(X, p, c) * (R(1) + R(1.5f) / (vx * vx));
+    }
+    if (VX > (double)MARFA_SQRT14_F) {
+        int I;
+        if constexpr (sizeof(R) == 8) I = (int)(VX / 0.5 - (double)1.00001f);
+        else I = (int)((float)VX / 0.5f - 1.00001f);
+        R vx = (R)VX;
+        R F = R(2) * (R(ASYM_W[I - 1]) * (R(ASYM_U[I]) - vx) + R(ASYM_W[I]) * (vx - R(ASYM_U[I - 1])));
+        return doppler<R>(X, p) + p.lorc / (X * X) * (R(1) + F);
+    }
+    return doppler<R>(X, p);
+}
+
+// Profile value at signed distance X64 (FP64) from the shifted centre.
+template <typename R>
+__device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl &c) {
+    const R X = (R)X64;
+    if (c.shape_id == 1) return chilor<R>(X, p, c);  // lorentzShape
+    const R ax = X < R(0) ? -X : X;
+    // fast path: clearly in region 0 (pure Lorentz x chi); margin covers the FP32 error of VX
+    const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
+    if (ax * p.xs >= margin) return chilor<R>(X, p, c);
+    return voigt_slow<R>(X64, p, c);
+}
+
+// ---------------------------------------------------------------- accumulators
+// FP64: plain sums.  FP32: Kahan-compensated pairs (value s, compensation c; the
+// compensated sum is s - c).
+template <typename R> struct Acc;
+template <> struct Acc<double> {
+    double s;
+    __device__ inline void zero() { s = 0.0; }
+    __device__ inline void add(double y) { s += y; }
+    __device__ inline double value() const { return s; }
+};
+template <> struct Acc<float> {
+    float s, c;
+    __device__ inline void zero() { s = 0.f; c = 0.f; }
+    __device__ inline void add(float y) {
+        float yy = y - c;
+        float t = s + yy;
+        c = (t - s) - yy;
+        s = t;
+    }
+    __device__ inline double value() const { return (double)s - (double)c; }
+};
+
+}  // namespace marfa

@@ -1,0 +1,1263 @@
+// marfa_lbl.hip -- MI355X (gfx950) line-by-line PT-table core behind include/marfa.h.
+//
+// Pipeline for a batch of atmospheric levels (one stream, no atomics in any sum):
+//   1. spectroscopy_kernel  one thread per (line, level): shifted centre, Doppler and
+//      Lorentz HWHM, S(T) in FP64 (src/Spectroscopy.f90:38-145, src/LBL.f90:64-73).
+//   2. per-level radix sort of the shifted centres (the gather needs lines ordered by
+//      their position inside each subinterval).
+//   3. coarse_kernel  one workgroup per record (level, 10 cm-1 subinterval): every line of
+//      the record's window is evaluated at the 21 positions of the 1 cm-1 grid and the
+//      samples are summed per (cell, edge) in registers, then reduced across the
+//      workgroup in a fixed order.  This is the level-0 part of leftLBL / centerLBL /
+//      rightLBL (src/LineGridCalc.f90) and carries ~90% of all samples.
+//   4. fine_kernel  one workgroup per (record, 1 cm-1 tile): every node of grids 1..9 and
+//      every fine point of the tile gathers the lines that deposit into it -- the
+//      inverse of the reference's walks, in closed form -- from LDS-staged candidates
+//      sorted by position; each node is owned by one thread, so sums run in a fixed
+//      order.  The tile then runs the cascade (src/Grids.f90:80-139) in FP64 from its
+//      level-0 cell down to the 2,048 points of the record it owns.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/marfa.h"
+#include "lbl_device.h"
+
+using namespace marfa;
+
+// WISO molar masses (src/MolarMasses.f90:140-146), bit patterns of the reference's values
+__constant__ static const uint32_t WISO_BITS[124] = {
+    0x419015A3U, 0x41A01E55U, 0x41981E45U, 0x41982249U, 0x41A82AFAU, 0x41A02AEBU,
+    0x41A00000U, 0x41B00000U, 0x41A80000U, 0x422FF596U, 0x4233F905U, 0x4237F9EFU,
+    0x4233F9E7U, 0x423BFD5FU, 0x4237FD56U, 0x423FFE48U, 0x423BFE40U, 0x42380000U,
+    0x42440075U, 0x42400000U, 0x423C0000U, 0x42300000U, 0x423FF07AU, 0x4247F3A0U,
+    0x4247F3A0U, 0x4243F390U, 0x4243F390U, 0x42500000U, 0x42500000U, 0x424C0000U,
+    0x424C0000U, 0x424C0000U, 0x42480000U, 0x42480000U, 0x42580000U, 0x42540000U,
+    0x42540000U, 0x42500000U, 0x42500000U, 0x424C0000U, 0x423000B0U, 0x4233FE2CU,
+    0x4233FE2CU, 0x423805C9U, 0x423405C6U, 0x41DFF5B8U, 0x41E7FBB6U, 0x41EFFF8AU,
+    0x41E7FF7AU, 0x41F80524U, 0x41F00514U, 0x41F00000U, 0x42000000U, 0x42040000U,
+    0x41804030U, 0x41884710U, 0x41884C86U, 0x4190539AU, 0x41FFEB3DU, 0x4207F9EFU,
+    0x4203F9E7U, 0x42100000U, 0x420C0000U, 0x42080000U, 0x41EFFBC9U, 0x41F7F6C2U,
+    0x42000249U, 0x427FD895U, 0x4283EA6EU, 0x4237F8EEU, 0x41883660U, 0x41903060U,
+    0x427BFB92U, 0x41880B39U, 0x419811E4U, 0x41900916U, 0x420FE80BU, 0x4217E4A3U,
+    0x429FDA3DU, 0x42A3D93EU, 0x42FFD335U, 0x424BDAE6U, 0x4253D768U, 0x426FDE15U,
+    0x4277DACFU, 0x42733E3AU, 0x42733DDCU, 0x42773E53U, 0x41F01590U, 0x41F81B6EU,
+    0x42000ED9U, 0x424FE2DFU, 0x4257DFA1U, 0x41E00C96U, 0x41D8164AU, 0x41E01D2BU,
+    0x41E0103DU, 0x4247FB11U, 0x424FF796U, 0x42080167U, 0x41D01007U, 0x41D81B6EU,
+    0x41F01809U, 0x420800A2U, 0x4283FBE1U, 0x4211F988U, 0x420FE14BU, 0x420BF9B9U,
+    0x42380167U, 0x4203FD94U, 0x417FEB3DU, 0x42C1E9D8U, 0x42C5E845U, 0x41EFFBC9U,
+    0x42BFD792U, 0x42C3D6ADU, 0x41E01007U, 0x41E81B6EU, 0x42001AD7U, 0x42BBE1E7U,
+    0x42BFE10DU, 0x42241AACU, 0x42AFFCA7U, 0x4289F2A5U,
+};
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int TILES = 10;      // 1 cm-1 tiles per 10 cm-1 record
+constexpr int CH = BLOCK;      // candidate lines staged per chunk
+constexpr int NCOARSE = 32;    // RK1, P[10], M[10], L[10], pad
+constexpr int NACC = 5114;     // per tile: 3*(2+4+...+512) grid nodes + 2048 fine points
+constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes
+constexpr int NNODES = NLVL_NODES + 2048;
+
+enum : uint8_t { CLS_NONE = 0, CLS_LEFT = 1, CLS_CENTER = 2, CLS_RIGHT = 3 };
+
+// Sortable integer image of a double (total order, also for negative values) and back.
+__host__ __device__ inline uint64_t key_of(double x) {
+    uint64_t b = __builtin_bit_cast(uint64_t, x);
+    return (b & 0x8000000000000000ULL) ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double double_of(uint64_t k) {
+    uint64_t b = (k & 0x8000000000000000ULL) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
+    return __builtin_bit_cast(double, b);
+}
+
+// Kernel arguments shared by the deposit kernels.
+struct KArgs {
+    int nlev, nrec;            // levels in this batch, subintervals per level
+    int lev0;                  // first level of the batch (output / level arrays)
+    int64_t lmin, nl;          // line range [lmin, lmin+nl) of the batch's arrays
+    const double *nus, *gD, *gL, *S;          // [nlev][nl]
+    const uint64_t *skey;                     // [nlev][nl] sorted keys of shifted centres
+    const int32_t *sidx;                      // [nlev][nl] line index of sorted entry
+    const int64_t *lo, *hi;                   // [nrec] windows (global line indices)
+    const double *org;                        // [nrec] subinterval origins
+    const double *sscale;                     // [lev] sample scale (power of 2)
+    const float *T;                           // [lev]
+    const float *outmul;                      // [lev] 1 (ACS) or density (VAC)
+    float cutoff;
+    int shape_id, chi_id, mol;
+    double *coarse;                           // [nlev*nrec][NCOARSE]
+    float *out;                               // [lev][nrec][NT] (absolute level index)
+    double *grids;  // optional: pre-cascade sums of record 0 in the reference's layout
+                    // (grids 0..9 as P|mid|L blocks, then RK(1..20481)), sample scale removed
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a contiguous
+// range of work items so that neighbouring records (which share 90+% of their line
+// windows) are processed under the same L2.  The grid is padded to 8*per blocks.
+__device__ inline int xcd_item(int nitems) {
+    const int per = (nitems + 7) / 8;
+    const int b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
+
+__device__ inline ShapeCtl make_ctl(const KArgs &a, int lev) {
+    ShapeCtl c;
+    c.shape_id = a.shape_id;
+    c.chi_id = a.chi_id;
+    c.mol = a.mol;
+    const float T = a.T[lev];
+    c.B1 = 0.0888f - 0.16f * expf(-(0.00410f * T));
+    c.B2 = 0.0526f * expf(-(0.00152f * T));
+    c.B1d = (double)0.0888f - (double)0.16f * exp(-((double)0.00410f * (double)T));
+    c.B2d = (double)0.0526f * exp(-((double)0.00152f * (double)T));
+    return c;
+}
+
+// ------------------------------------------------------------------ 1. spectroscopy
+struct SpecArgs {
+    int nlev;
+    int64_t lmin, nl;
+    const double *nu;
+    const float *S, *gair, *gself, *elow, *nT, *delta;
+    const int32_t *molIso;
+    const float *P, *T;          // [nlev] (batch-relative)
+    const double *pself;         // [nlev]
+    const float *tips;           // [74][501]
+    double *nus, *gD, *gL, *ST;  // [nlev][nl]
+    uint64_t *keys;              // [nlev][nl] sortable bits of nus
+    int32_t *idx;                // [nlev][nl]
+};
+
+__global__ __launch_bounds__(BLOCK) void spectroscopy_kernel(SpecArgs a) {
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int lev = blockIdx.y;
+    if (j >= a.nl) return;
+    const int64_t i = a.lmin + j;
+    const float P = a.P[lev], T = a.T[lev];
+    const double dT = T, dP = P, ps = a.pself[lev];
+    const int iso = a.molIso[i] / 100;
+    const double M = (double)bits_f(WISO_BITS[iso - 1]);
+    const double nu = a.nu[i];
+    // shifted centre: FP32 product delta*P as src/Spectroscopy.f90:48 forms it, so that
+    // every subinterval's line selection is the reference's
+    const double nus = nu + (double)(a.delta[i] * P);
+    const double gD = ((double)MARFA_DOPC_F * nus) * sqrt(dT / M);
+    const double gL = pow((double)MARFA_TREF_F / dT, (double)a.nT[i]) *
+                      ((double)a.gair[i] * (dP - ps) + (double)a.gself[i] * ps);
+    // TIPS(T) by linear interpolation, src/Spectroscopy.f90:110-114
+    const int ti = (int)((dT - 20.0) / 2.0 + 1.0);
+    const double wu = (dT - ((double)ti * 2.0 + 18.0)) / 2.0;
+    const float *row = a.tips + (iso - 1) * 501;
+    const double Q = (1.0 - wu) * (double)row[ti - 1] + wu * (double)row[ti];
+    const double c2 = (double)MARFA_C2_F, E = a.elow[i];
+    const double tf = (double)row[138] / Q;
+    const double bf = exp(-(c2 * E / dT)) / exp(-(c2 * E / (double)MARFA_TREF_F));
+    const double ef = (1.0 - exp(-((c2 * nu) / dT))) / (1.0 - exp(-((c2 * nu) / (double)MARFA_TREF_F)));
+    const double ST = (double)a.S[i] * tf * bf * ef;
+    const int64_t o = (int64_t)lev * a.nl + j;
+    a.nus[o] = nus;
+    a.gD[o] = gD;
+    a.gL[o] = gL;
+    a.ST[o] = ST;
+    a.keys[o] = key_of(nus);
+    a.idx[o] = (int32_t)i;
+}
+
+// per-level maximum exponent of S(T) (FP32 form scales samples by a power of two so the
+// strongest line sits near 1; max is order-independent, so the atomic is deterministic)
+__global__ __launch_bounds__(BLOCK) void smax_kernel(const double *ST, int64_t nl, int *emax) {
+    const int lev = blockIdx.y;
+    int e = -2000;
+    for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < nl; j += (int64_t)gridDim.x * BLOCK) {
+        double s = ST[(int64_t)lev * nl + j];
+        if (s > 0.0) e = max(e, ilogb(s));
+    }
+    for (int off = 32; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(emax + lev, e);
+}
+
+// ------------------------------------------------------------------ 3. coarse (level 0)
+// Bit q of a mask marks position q*0.5 cm-1 of the subinterval (q = 0..20).
+constexpr uint32_t EVEN = 0x155555u, ODD = 0x0AAAAAu;
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
+    const int nitems = a.nlev * a.nrec;
+    const int r = xcd_item(nitems);
+    if (r >= nitems) return;
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const double org = a.org[s];
+    const int64_t lo = a.lo[s], hi = a.hi[s];
+    const double cut = (double)a.cutoff;
+    const double top = org + 10.0;  // endDeltaWV
+    const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
+    const double sc = a.sscale[a.lev0 + lev];
+
+    Acc<R> aR, aP[10], aM[10], aL[10];
+    aR.zero();
+#pragma unroll
+    for (int c = 0; c < 10; c++) { aP[c].zero(); aM[c].zero(); aL[c].zero(); }
+
+    const int64_t base = (int64_t)lev * a.nl - a.lmin;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) {
+        const double nus = a.nus[base + i];
+        const double u = nus - org;
+        uint32_t mP = 0, mM = 0, mL = 0;
+        bool use = false;
+        if (nus < org) {                 // leftLBL, src/LineGridCalc.f90:21-33
+            const double d = -u;
+            if (!(d > cut)) {
+                use = true;
+                if (d >= 1.0) { mP = EVEN & 0x7FFFFu; mM = ODD; mL = EVEN & ~1u; }
+                else { mP = EVEN & 0x7FFFCu; mM = ODD & ~2u; mL = EVEN & ~5u; }
+            }
+        } else if (nus > top) {          // rightLBL, :508-518
+            const double dr = u - 10.0;
+            if (!(dr >= cut)) {
+                use = true;
+                if (dr >= 1.0) { mP = EVEN & 0x7FFFFu; mM = ODD; mL = EVEN & ~1u; }
+                else { mP = EVEN & 0x1FFFFu; mM = ODD & 0x3FFFFu; mL = EVEN & 0x7FFFCu; }
+            }
+        } else if (u < 10.0) {           // centerLBL level-0 walks, :201-214 and :343-358
+            use = true;
+            const int e0 = u >= 2.0 ? (int)floor(u) - 1 : 0;
+            const double v = 10.0 - u;
+            const int e0r = v >= 2.0 ? (int)floor(v) - 1 : 0;
+            if (e0 > 0) {
+                mP = EVEN & ((1u << (2 * e0 - 1)) - 1u);
+                mM = ODD & ((1u << (2 * e0)) - 1u);
+                mL = EVEN & ((1u << (2 * e0 + 1)) - 1u) & ~1u;
+            }
+            if (e0r > 0) {
+                const int qs = 20 - 2 * e0r;
+                mP |= EVEN & ~((1u << qs) - 1u) & 0x7FFFFu;
+                mM |= ODD & ~((1u << qs) - 1u);
+                mL |= EVEN & ~((1u << (qs + 1)) - 1u);
+            }
+        }
+        if (!use) continue;
+        const LineShape<R> p = make_line<R>(a.gD[base + i], a.gL[base + i], a.S[base + i] * sc);
+        const uint32_t need = mP | mM | mL | 1u;
+#pragma unroll
+        for (int q = 0; q <= 20; q++) {
+            // positions no role needs are moved far out, onto the cheap Lorentz branch
+            const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
+            const R val = shape_eval<R>(X, p, ctl);
+            if (q == 0) aR.add(val);
+            if (q & 1) {
+                aM[q >> 1].add(((mM >> q) & 1u) ? val : R(0));
+            } else {
+                if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : R(0));
+                if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : R(0));
+            }
+        }
+    }
+
+    // fixed-order reduction: wave butterflies, then the 4 waves in order, in FP64
+    __shared__ double red[BLOCK / 64][31];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double vals[31];
+    vals[0] = aR.value();
+#pragma unroll
+    for (int c = 0; c < 10; c++) {
+        vals[1 + c] = aP[c].value();
+        vals[11 + c] = aM[c].value();
+        vals[21 + c] = aL[c].value();
+    }
+#pragma unroll
+    for (int k = 0; k < 31; k++) {
+        double x = vals[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (lane == 0) red[wv][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 31) {
+        double x = red[0][threadIdx.x];
+        for (int w = 1; w < BLOCK / 64; w++) x += red[w][threadIdx.x];
+        a.coarse[(int64_t)r * NCOARSE + threadIdx.x] = x;
+    }
+}
+
+// ------------------------------------------------------------------ 4. fine + cascade
+__device__ inline int lb_ge(const double *x, int n, double v) {  // first j with x[j] >= v
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (x[m] < v) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+__device__ inline int lb_gt(const double *x, int n, double v) {  // first j with x[j] > v
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (!(x[m] > v)) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+__device__ inline int lb_vlt(const double *x, int n, double v) {  // first j with x[j] < v (x non-increasing)
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (!(x[m] < v)) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+
+// offset of level k's position nodes in the per-tile node list, k = 1..9
+__device__ inline int lvl_node_off(int k) { return (1 << (k + 1)) - 4 + (k - 1); }
+// offset of level k's P block in the tile accumulators; M at +2^k, L at +2*2^k
+__device__ inline int lvl_acc_off(int k) { return 3 * ((1 << k) - 2); }
+constexpr int FINE_OFF = 3066;
+
+template <typename R>
+struct FineShared {
+    Acc<R> acc[NACC];
+    double u[CH], v[CH];
+    LineShape<R> ln[CH];
+    uint8_t cls[CH];
+    int64_t crange[2];
+};
+
+template <typename R>
+__device__ inline void add_acc(FineShared<R> &sh, int slot, R val) { sh.acc[slot].add(val); }
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
+    __shared__ FineShared<R> sh;
+    const int nitems = a.nlev * a.nrec * TILES;
+    const int w = xcd_item(nitems);
+    if (w >= nitems) return;
+    const int r = w / TILES, t = w % TILES;
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const double org = a.org[s];
+    const int64_t wlo = a.lo[s], whi = a.hi[s];
+    const double cut = (double)a.cutoff;
+    const double top = org + 10.0;
+    const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
+    const double sc = a.sscale[a.lev0 + lev];
+    const int tid = threadIdx.x;
+
+    for (int e = tid; e < NACC; e += BLOCK) sh.acc[e].zero();
+    // candidate lines: shifted centre within [org + t - 2, org + t + 3] (every deposit of
+    // grids 1..9 and of the fine points into this tile comes from there)
+    const uint64_t *skey = a.skey + (int64_t)lev * a.nl;
+    if (tid < 2) {
+        const uint64_t x = key_of(tid == 0 ? org + (t - 2) - 1e-9 : org + (t + 3) + 1e-9);
+        int64_t lo = 0, hi = a.nl;
+        while (lo < hi) {
+            int64_t m = (lo + hi) >> 1;
+            if (skey[m] < x) lo = m + 1; else hi = m;
+        }
+        sh.crange[tid] = lo;
+    }
+    __syncthreads();
+    const int64_t c0 = sh.crange[0], c1 = sh.crange[1];
+    const int64_t base = (int64_t)lev * a.nl - a.lmin;
+
+    for (int64_t cb = c0; cb < c1; cb += CH) {
+        const int cn = (int)min((int64_t)CH, c1 - cb);
+        if (tid < cn) {
+            const int64_t j = cb + tid;
+            const double nus = double_of(skey[j]);
+            const int64_t li = a.sidx[(int64_t)lev * a.nl + j];
+            const double u = nus - org;
+            uint8_t cls = CLS_NONE;
+            if (li >= wlo && li < whi) {
+                if (nus < org) {
+                    const double d = -u;
+                    if (!(d > cut) && d < 1.0) cls = CLS_LEFT;
+                } else if (nus > top) {
+                    const double dr = u - 10.0;
+                    if (!(dr >= cut) && dr < 1.0) cls = CLS_RIGHT;
+                } else if (u < 10.0) {
+                    cls = CLS_CENTER;
+                }
+            }
+            sh.u[tid] = u;
+            sh.v[tid] = 10.0 - u;
+            sh.cls[tid] = cls;
+            if (cls != CLS_NONE) sh.ln[tid] = make_line<R>(a.gD[base + li], a.gL[base + li], a.S[base + li] * sc);
+        }
+        __syncthreads();
+        const double umin = sh.u[0], umax = sh.u[cn - 1];
+
+        for (int nd = tid; nd < NNODES; nd += BLOCK) {
+            if (nd < NLVL_NODES) {
+                // ---- grid level k, position q (spacing H_k/2)
+                int k = 1;
+                while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
+                const int qloc = nd - lvl_node_off(k);
+                const int span = 2 << k;              // positions per tile
+                const int q = t * span + qloc;
+                const int NTk = 10 << k;
+                const double Hk = hk(k), hq = 0.5 * Hk;
+                const double xq = q * hq;
+                const int cellbase = t << k;          // cells before the tile
+                const int accb = lvl_acc_off(k), nk = 1 << k;
+                // center lines, left walk: floor(u 2^k) in [nlo, nhi]
+                {
+                    const int nlo = max(2, ((q + 1) >> 1) + 1), nhi = 2 * (q >> 2) + 3;
+                    const double ulo = nlo * Hk, uhi = (nhi + 1) * Hk;
+                    if (umax >= ulo && umin < uhi) {
+                        const int j0 = lb_ge(sh.u, cn, ulo), j1 = lb_ge(sh.u, cn, uhi);
+                        for (int j = j0; j < j1; j++) {
+                            if (sh.cls[j] != CLS_CENTER) continue;
+                            const double u = sh.u[j];
+                            const int n = (int)floor(__builtin_ldexp(u, k));
+                            const int lo_q = 4 * max(0, (n >> 1) - 1), hi_q = 2 * (n - 1);
+                            if (q < lo_q || q > hi_q) continue;
+                            const R val = shape_eval<R>(u - xq, sh.ln[j], ctl);
+                            if (q & 1) {
+                                sh.acc[accb + nk + ((q + 1) >> 1) - 1 - cellbase].add(val);
+                            } else {
+                                if (q > lo_q && qloc > 0) sh.acc[accb + 2 * nk + (q >> 1) - 1 - cellbase].add(val);
+                                if (q < hi_q && qloc < span) sh.acc[accb + (q >> 1) - cellbase].add(val);
+                            }
+                        }
+                    }
+                }
+                // center lines, right walk: floor(v 2^k) in [nlo, nhi] for q' = 2 NT_k - q
+                {
+                    const int qp = 2 * NTk - q;
+                    const int nlo = max(2, ((qp + 1) >> 1) + 1), nhi = 2 * (qp >> 2) + 3;
+                    const double vlo = nlo * Hk, vhi = (nhi + 1) * Hk;
+                    // v = 10 - u is non-increasing along the chunk
+                    if (sh.v[0] >= vlo && sh.v[cn - 1] < vhi) {
+                        const int j0 = lb_vlt(sh.v, cn, vhi), j1 = lb_vlt(sh.v, cn, vlo);
+                        for (int j = j0; j < j1; j++) {
+                            if (sh.cls[j] != CLS_CENTER) continue;
+                            const double v = sh.v[j];
+                            const int n = (int)floor(__builtin_ldexp(v, k));
+                            const int lo_q = 2 * NTk - 2 * (n - 1), hi_q = 2 * NTk - 4 * max(0, (n >> 1) - 1);
+                            if (q < lo_q || q > hi_q) continue;
+                            const R val = shape_eval<R>(v - qp * hq, sh.ln[j], ctl);
+                            if (q & 1) {
+                                sh.acc[accb + nk + ((q + 1) >> 1) - 1 - cellbase].add(val);
+                            } else {
+                                if (q > lo_q && qloc > 0) sh.acc[accb + 2 * nk + (q >> 1) - 1 - cellbase].add(val);
+                                if (q < hi_q && qloc < span) sh.acc[accb + (q >> 1) - cellbase].add(val);
+                            }
+                        }
+                    }
+                }
+                // wing lines within H_{k-1} of the left edge: cells 1 and 2 (leftLBL :34-164)
+                if (t == 0 && q <= 4 && umin < 0.0) {
+                    const double hkm1 = hk(k - 1);
+                    const int j0 = lb_gt(sh.u, cn, -hkm1), j1 = lb_ge(sh.u, cn, 0.0);
+                    for (int j = j0; j < j1; j++) {
+                        if (sh.cls[j] != CLS_LEFT) continue;
+                        const double u = sh.u[j];
+                        const bool eq = (-u >= Hk);  // m == k (else m > k)
+                        if (!eq && q < 2) continue;
+                        const R val = shape_eval<R>(u - xq, sh.ln[j], ctl);
+                        // cells 1 and 2 are local cells 0 and 1 of tile 0
+                        if (q == 0) sh.acc[accb + 0].add(val);
+                        else if (q == 1) sh.acc[accb + nk + 0].add(val);
+                        else if (q == 2) {
+                            if (eq) sh.acc[accb + 2 * nk + 0].add(val);
+                            sh.acc[accb + 1].add(val);
+                        } else if (q == 3) sh.acc[accb + nk + 1].add(val);
+                        else sh.acc[accb + 2 * nk + 1].add(val);
+                    }
+                }
+                // wing lines within H_{k-1} of the right edge: cells NT_k and NT_k - 1
+                if (t == TILES - 1 && 2 * NTk - q <= 4 && umax > 10.0 - 1e-9) {
+                    const int qp = 2 * NTk - q;
+                    const double hkm1 = hk(k - 1);
+                    const int j0 = lb_gt(sh.u, cn, 10.0 - 1e-9);
+                    for (int j = j0; j < cn; j++) {
+                        if (sh.cls[j] != CLS_RIGHT) continue;
+                        const double dr = sh.u[j] - 10.0;
+                        if (!(dr < hkm1)) break;
+                        const bool eq = (dr >= Hk);
+                        if (!eq && qp < 2) continue;
+                        const R val = shape_eval<R>(dr + qp * hq, sh.ln[j], ctl);
+                        const int cN = nk - 1, cN1 = nk - 2;  // local cells of NT_k, NT_k - 1
+                        if (qp == 0) sh.acc[accb + 2 * nk + cN].add(val);
+                        else if (qp == 1) sh.acc[accb + nk + cN].add(val);
+                        else if (qp == 2) {
+                            if (eq) sh.acc[accb + cN].add(val);
+                            sh.acc[accb + 2 * nk + cN1].add(val);
+                        } else if (qp == 3) sh.acc[accb + nk + cN1].add(val);
+                        else sh.acc[accb + cN1].add(val);
+                    }
+                }
+            } else {
+                // ---- fine point i (1-based record index), position (i-1) H
+                const int il = nd - NLVL_NODES;
+                const int i = 2048 * t + 2 + il;
+                const double xi = (double)(i - 1) * HF;
+                Acc<R> &acc = sh.acc[FINE_OFF + il];
+                // centre lines, left part: i in [4 e9 + 2, floor(u/H) + 1] (centerLBL :332-339)
+                {
+                    const double uhi = (double)(((i - 2) >> 2) + 2) * H9;
+                    if (umax >= xi && umin < uhi) {
+                        const int j0 = lb_ge(sh.u, cn, xi), j1 = lb_ge(sh.u, cn, uhi);
+                        for (int j = j0; j < j1; j++)
+                            if (sh.cls[j] == CLS_CENTER) acc.add(shape_eval<R>(sh.u[j] - xi, sh.ln[j], ctl));
+                    }
+                }
+                // right part: i in [floor(u/H) + 2, NT - 4 e9'] (centerLBL :485-490)
+                {
+                    const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
+                    if (umin < xi && sh.v[cn - 1] < vhi) {
+                        const int j0 = lb_vlt(sh.v, cn, vhi), j1 = lb_ge(sh.u, cn, xi);
+                        for (int j = j0; j < j1; j++)
+                            if (sh.cls[j] == CLS_CENTER) acc.add(shape_eval<R>(sh.u[j] - xi, sh.ln[j], ctl));
+                    }
+                }
+                // wing lines closer than H9 to an edge: four fine points (leftLBL :107-112,
+                // rightLBL :592-595)
+                if (t == 0 && i <= 5 && umin < 0.0) {
+                    const int j0 = lb_gt(sh.u, cn, -H9), j1 = lb_ge(sh.u, cn, 0.0);
+                    for (int j = j0; j < j1; j++) {
+                        if (sh.cls[j] != CLS_LEFT) continue;
+                        const double uu = sh.u[j];
+                        const double X = i == 2 ? uu - HF : i == 3 ? (uu - HF) - HF
+                                       : i == 4 ? (uu + HF) - H9 : uu - H9;
+                        acc.add(shape_eval<R>(X, sh.ln[j], ctl));
+                    }
+                }
+                if (t == TILES - 1 && i >= NT - 3 && umax > 10.0 - 1e-9) {
+                    const int j0 = lb_gt(sh.u, cn, 10.0 - 1e-9);
+                    for (int j = j0; j < cn; j++) {
+                        if (sh.cls[j] != CLS_RIGHT) continue;
+                        const double dr = sh.u[j] - 10.0;
+                        if (!(dr < H9)) break;
+                        const double X = i == NT ? dr : i == NT - 1 ? dr + HF
+                                       : i == NT - 2 ? (dr + HF) + HF : (dr + H9) - HF;
+                        acc.add(shape_eval<R>(X, sh.ln[j], ctl));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- cascade of this tile in FP64 (src/Grids.f90:80-139)
+    double *g = reinterpret_cast<double *>(sh.acc);
+    static_assert(sizeof(Acc<R>) == sizeof(double), "accumulator slot must be 8 bytes");
+    for (int e = tid; e < NACC; e += BLOCK) {
+        const double v = sh.acc[e].value();
+        __builtin_memcpy(&g[e], &v, 8);
+    }
+    __syncthreads();
+    const double *cz = a.coarse + (int64_t)r * NCOARSE;
+    if (a.grids && r == 0) {
+        for (int e = tid; e < NACC; e += BLOCK) {
+            int dst;
+            if (e >= FINE_OFF) {
+                dst = 30690 + 2048 * t + 1 + (e - FINE_OFF);
+            } else {
+                int k = 1;
+                while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
+                const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
+                dst = 30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl;
+            }
+            a.grids[dst] = g[e] / sc;
+        }
+        if (tid < 3) {
+            const int dst = tid * 10 + t;  // level-0 P|mid|L of cell t+1
+            a.grids[dst] = cz[1 + 10 * tid + t] / sc;
+        }
+        if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
+    }
+    auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i) {
+        fP[i] += P;
+        double A = 0.375 * P + 0.75 * M - 0.125 * L;
+        if (A < P || A < M) A = 0.5 * (P + M);
+        fM[i] += A;
+        fL[i] += M;
+        fP[i + 1] += M;
+        A = 0.375 * L + 0.75 * M - 0.125 * P;
+        if (A < L || A < M) A = 0.5 * (L + M);
+        fM[i + 1] += A;
+        fL[i + 1] += L;
+    };
+    if (tid == 0) {
+        const int k1 = lvl_acc_off(1);
+        step(cz[1 + t], cz[11 + t], cz[21 + t], g + k1, g + k1 + 2, g + k1 + 4, 0);
+    }
+    __syncthreads();
+    for (int k = 1; k < 9; k++) {
+        const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
+        for (int c = tid; c < nk; c += BLOCK)
+            step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c);
+        __syncthreads();
+    }
+    {
+        const int cb = lvl_acc_off(9);
+        double *rk = g + FINE_OFF;
+        for (int c = tid; c < 512; c += BLOCK) {
+            const double P = g[cb + c], M = g[cb + 512 + c], L = g[cb + 1024 + c];
+            double A = P * 0.375 + M * 0.75 - L * 0.125;
+            if (A < P || A < M) A = 0.5 * (P + M);
+            rk[4 * c] += A;
+            rk[4 * c + 1] += M;
+            A = L * 0.375 + M * 0.75 - P * 0.125;
+            if (A < L || A < M) A = 0.5 * (L + M);
+            rk[4 * c + 2] += A;
+            rk[4 * c + 3] += L;
+        }
+    }
+    __syncthreads();
+    const double mul = (double)a.outmul[a.lev0 + lev] / sc;
+    float *out = a.out + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
+    for (int e = tid; e < 2048; e += BLOCK) out[2048 * t + 1 + e] = (float)(g[FINE_OFF + e] * mul);
+    if (t == 0 && tid == 0) out[0] = (float)(cz[0] * mul);
+}
+
+// ------------------------------------------------------------------ sample counts
+__global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
+    const int nitems = a.nlev * a.nrec;
+    const int r = blockIdx.x;
+    if (r >= nitems) return;
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const double org = a.org[s];
+    const double cut = (double)a.cutoff;
+    const int64_t base = (int64_t)lev * a.nl - a.lmin;
+    unsigned long long cnt = 0;
+    for (int64_t i = a.lo[s] + threadIdx.x; i < a.hi[s]; i += BLOCK) {
+        const double nus = a.nus[base + i], u = nus - org;
+        if (nus < org) {
+            const double d = -u;
+            if (d > cut) continue;
+            int m = 0;
+            while (m <= 9 && d < hk(m)) m++;
+            cnt += 21 + 2 * m;
+        } else if (nus > org + 10.0) {
+            const double dr = u - 10.0;
+            if (dr >= cut) continue;
+            int m = 0;
+            while (m <= 9 && dr < hk(m)) m++;
+            cnt += 21 + 2 * m;
+        } else if (u < 10.0) {
+            const double v = 10.0 - u;
+            long long cells = 0;
+            int ep = 0, epr = 0;
+            for (int k = 0; k <= 9; k++) {
+                const int e = u >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(u, k)) - 1 : 0;
+                const int er = v >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(v, k)) - 1 : 0;
+                cells += (e - 2 * ep) + (er - 2 * epr);
+                ep = e;
+                epr = er;
+            }
+            const long long fu = (long long)floor(u / HF);
+            long long fine = 0;
+            if (u >= HF) fine += fu - 4LL * ep;
+            const long long rlo = fu + 2, rhi = NT - 4LL * epr;
+            if (rhi >= rlo) fine += rhi - rlo + 1;
+            cnt += 2 + 2 * cells + fine;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    __shared__ unsigned long long red[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = 0;
+        for (int w = 0; w < BLOCK / 64; w++) x += red[w];
+        counts[r] = x;
+    }
+}
+
+}  // namespace
+
+// ====================================================================== host side
+struct marfa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // inputs
+    int64_t n = 0;
+    std::vector<double> h_nu;
+    double *d_nu = nullptr;
+    float *d_S = nullptr, *d_gair = nullptr, *d_gself = nullptr, *d_elow = nullptr, *d_nT = nullptr,
+          *d_delta = nullptr;
+    int32_t *d_molIso = nullptr;
+    float *d_tips = nullptr;
+    bool tips_set = false;
+    std::vector<float> hP, hT, hrho;
+    // work buffers
+    struct Buf {
+        void *p = nullptr;
+        size_t bytes = 0;
+    };
+    Buf params[4], keys[2], idx[2], sorttmp, coarse, lvl, win, out, counts;
+    hipEvent_t ev[5] = {};
+    float timings[5] = {0, 0, 0, 0, 0};
+    double *dbg_grids = nullptr;
+};
+
+namespace {
+
+int set_err(marfa_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err(c, MARFA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(marfa_ctx *c, marfa_ctx::Buf &b, size_t bytes) {
+    if (b.bytes >= bytes) return 0;
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = std::max(bytes, (size_t)256);
+    HIPCHK(hipMalloc(&b.p, want));
+    b.bytes = want;
+    return 0;
+}
+
+struct Windows {
+    int nrec = 0;
+    std::vector<int64_t> lo, hi;
+    std::vector<int32_t> rec;
+    std::vector<double> org;
+};
+
+// app/main.f90:117-174 with src/LBL.f90:39-61 (window = lines read before the exit test
+// fires; carry = last line at or below capWV) and the starting line of app/main.f90:223-240.
+void compute_windows(const std::vector<double> &nu, const marfa_job &j, Windows &w) {
+    const int64_t n = (int64_t)nu.size();
+    const double cut = (double)j.cutoff;
+    int64_t start = 0;
+    if (j.start_wv > cut) {
+        const double ext = j.start_wv - cut;
+        start = std::upper_bound(nu.begin(), nu.end(), ext) - nu.begin();
+    }
+    const double startWV = start < n ? nu[start] : INFINITY;
+    int64_t idx = start;
+    w = Windows();
+    for (double a = j.start_wv; a < j.end_wv; a = a + 10.0) {
+        const double extStart = a - cut, extEnd = a + 10.0 + cut;
+        double cap = extStart + 10.0;
+        if (cap <= startWV) cap = startWV;
+        int64_t hi = std::lower_bound(nu.begin() + std::min(idx, n), nu.end(), extEnd) - nu.begin();
+        hi = std::max(hi, idx);
+        int64_t ub = std::upper_bound(nu.begin() + std::min(idx, n), nu.begin() + hi, cap) - nu.begin();
+        int64_t carry = ub - 1 >= idx ? ub - 1 : idx;
+        w.lo.push_back(idx);
+        w.hi.push_back(hi);
+        w.rec.push_back((int32_t)((a + 1.0) / 10.0));
+        w.org.push_back(a);
+        idx = carry;
+        w.nrec++;
+    }
+}
+
+struct LevelSet {  // host-side level state for one run
+    std::vector<float> P, T, outmul;
+    std::vector<double> pself;
+};
+
+// Run levels x records.  Windows are level-independent; out is [nlev][nrec][NT].
+int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
+        hipStream_t st, bool timed) {
+    const int nlev = (int)L.P.size(), nrec = W.nrec;
+    if (nrec == 0 || nlev == 0) return 0;
+    for (float T : L.T)
+        if (!(T >= 20.f && T < 1018.f)) return set_err(c, MARFA_E_TEMPERATURE, "temperature outside the TIPS table");
+    const int64_t lmin = *std::min_element(W.lo.begin(), W.lo.end());
+    const int64_t lmax = *std::max_element(W.hi.begin(), W.hi.end());
+    const int64_t nl = std::max<int64_t>(lmax - lmin, 1);
+    if (lmin < 0 || lmax > c->n) return set_err(c, MARFA_E_WINDOW, "window outside the line list");
+
+    // level batches so that per-(line, level) arrays stay within a memory budget
+    const size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4);
+    const size_t budget = (size_t)24 << 30;
+    int lb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nlev, budget / std::max<size_t>(per_level, 1)));
+
+    // level arrays [nlev]: P, T, outmul (float), pself, sscale (double)
+    if (ensure(c, c->lvl, (size_t)nlev * (3 * 4 + 2 * 8) + 64)) return MARFA_E_HIP;
+    float *dP = (float *)c->lvl.p, *dT = dP + nlev, *dmul = dT + nlev;
+    double *dps = (double *)(((uintptr_t)(dmul + nlev) + 15) & ~(uintptr_t)15), *dsc = dps + nlev;
+    HIPCHK(hipMemcpyAsync(dP, L.P.data(), nlev * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dT, L.T.data(), nlev * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dmul, L.outmul.data(), nlev * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
+    // windows
+    if (ensure(c, c->win, (size_t)nrec * 24 + 64)) return MARFA_E_HIP;
+    int64_t *dlo = (int64_t *)c->win.p, *dhi = dlo + nrec;
+    double *dorg = (double *)(dhi + nrec);
+    HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dhi, W.hi.data(), nrec * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dorg, W.org.data(), nrec * 8, hipMemcpyHostToDevice, st));
+
+    const size_t arr = (size_t)lb * nl;
+    for (int q = 0; q < 4; q++)
+        if (ensure(c, c->params[q], arr * 8)) return MARFA_E_HIP;
+    for (int q = 0; q < 2; q++) {
+        if (ensure(c, c->keys[q], arr * 8)) return MARFA_E_HIP;
+        if (ensure(c, c->idx[q], arr * 4)) return MARFA_E_HIP;
+    }
+    if (ensure(c, c->coarse, (size_t)lb * nrec * NCOARSE * 8)) return MARFA_E_HIP;
+    size_t tmpb = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                       (int32_t *)nullptr, (int32_t *)nullptr, (int)nl, 0, 64, st);
+    if (ensure(c, c->sorttmp, tmpb)) return MARFA_E_HIP;
+
+    // FP32 form: per-level power-of-two sample scale; FP64 form: 1
+    std::vector<double> hsc(nlev, 1.0);
+    int *demax = nullptr;
+    if (job.precision == MARFA_FP32K) {
+        if (ensure(c, c->counts, (size_t)nlev * 4 + 64)) return MARFA_E_HIP;
+        demax = (int *)c->counts.p;
+        std::vector<int> init(nlev, -2000);
+        HIPCHK(hipMemcpyAsync(demax, init.data(), nlev * 4, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(dsc, hsc.data(), nlev * 8, hipMemcpyHostToDevice, st));
+
+    float tms[5] = {0, 0, 0, 0, 0};
+    for (int l0 = 0; l0 < nlev; l0 += lb) {
+        const int nb = std::min(lb, nlev - l0);
+        if (timed) HIPCHK(hipEventRecord(c->ev[0], st));
+        SpecArgs sa;
+        sa.nlev = nb;
+        sa.lmin = lmin;
+        sa.nl = nl;
+        sa.nu = c->d_nu;
+        sa.S = c->d_S;
+        sa.gair = c->d_gair;
+        sa.gself = c->d_gself;
+        sa.elow = c->d_elow;
+        sa.nT = c->d_nT;
+        sa.delta = c->d_delta;
+        sa.molIso = c->d_molIso;
+        sa.P = dP + l0;
+        sa.T = dT + l0;
+        sa.pself = dps + l0;
+        sa.tips = c->d_tips;
+        sa.nus = (double *)c->params[0].p;
+        sa.gD = (double *)c->params[1].p;
+        sa.gL = (double *)c->params[2].p;
+        sa.ST = (double *)c->params[3].p;
+        sa.keys = (uint64_t *)c->keys[0].p;
+        sa.idx = (int32_t *)c->idx[0].p;
+        dim3 gs((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
+        hipLaunchKernelGGL(spectroscopy_kernel, gs, dim3(BLOCK), 0, st, sa);
+        HIPCHK(hipGetLastError());
+        if (job.precision == MARFA_FP32K) {
+            dim3 gm((unsigned)std::min<int64_t>((nl + BLOCK - 1) / BLOCK, 1024), (unsigned)nb);
+            hipLaunchKernelGGL(smax_kernel, gm, dim3(BLOCK), 0, st, (const double *)sa.ST, nl, demax + l0);
+            HIPCHK(hipGetLastError());
+            std::vector<int> em(nb);
+            HIPCHK(hipMemcpyAsync(em.data(), demax + l0, nb * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int q = 0; q < nb; q++) hsc[l0 + q] = em[q] > -2000 ? std::ldexp(1.0, -em[q]) : 1.0;
+            HIPCHK(hipMemcpyAsync(dsc + l0, hsc.data() + l0, nb * 8, hipMemcpyHostToDevice, st));
+        }
+        if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
+        for (int q = 0; q < nb; q++) {
+            size_t tb = c->sorttmp.bytes;
+            hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+                c->sorttmp.p, tb, (uint64_t *)c->keys[0].p + (size_t)q * nl,
+                (uint64_t *)c->keys[1].p + (size_t)q * nl, (int32_t *)c->idx[0].p + (size_t)q * nl,
+                (int32_t *)c->idx[1].p + (size_t)q * nl, (int)nl, 0, 64, st);
+            if (e != hipSuccess) return set_err(c, MARFA_E_HIP, std::string("radix sort: ") + hipGetErrorString(e));
+        }
+        if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
+        KArgs ka;
+        ka.nlev = nb;
+        ka.nrec = nrec;
+        ka.lev0 = l0;
+        ka.lmin = lmin;
+        ka.nl = nl;
+        ka.nus = sa.nus;
+        ka.gD = sa.gD;
+        ka.gL = sa.gL;
+        ka.S = sa.ST;
+        ka.skey = (const uint64_t *)c->keys[1].p;
+        ka.sidx = (const int32_t *)c->idx[1].p;
+        ka.lo = dlo;
+        ka.hi = dhi;
+        ka.org = dorg;
+        ka.sscale = dsc;
+        ka.T = dT;
+        ka.outmul = dmul;
+        ka.cutoff = job.cutoff;
+        ka.shape_id = job.shape_id;
+        ka.chi_id = job.chi_id;
+        ka.mol = job.mol_code;
+        ka.coarse = (double *)c->coarse.p;
+        ka.out = out;
+        ka.grids = c->dbg_grids;
+        const int nitems = nb * nrec;
+        const unsigned gcoarse = (unsigned)(8 * ((nitems + 7) / 8));
+        const unsigned gfine = (unsigned)(8 * ((nitems * TILES + 7) / 8));
+        if (job.precision == MARFA_FP32K)
+            hipLaunchKernelGGL(coarse_kernel<float>, dim3(gcoarse), dim3(BLOCK), 0, st, ka);
+        else
+            hipLaunchKernelGGL(coarse_kernel<double>, dim3(gcoarse), dim3(BLOCK), 0, st, ka);
+        HIPCHK(hipGetLastError());
+        if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
+        if (job.precision == MARFA_FP32K)
+            hipLaunchKernelGGL(fine_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka);
+        else
+            hipLaunchKernelGGL(fine_kernel<double>, dim3(gfine), dim3(BLOCK), 0, st, ka);
+        HIPCHK(hipGetLastError());
+        if (timed) {
+            HIPCHK(hipEventRecord(c->ev[4], st));
+            HIPCHK(hipEventSynchronize(c->ev[4]));
+            float x;
+            for (int q = 0; q < 4; q++) {
+                HIPCHK(hipEventElapsedTime(&x, c->ev[q], c->ev[q + 1]));
+                tms[q] += x;
+            }
+        }
+    }
+    if (timed) {
+        tms[4] = tms[0] + tms[1] + tms[2] + tms[3];
+        memcpy(c->timings, tms, sizeof tms);
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int marfa_version(void) { return 1; }
+
+const char *marfa_last_error(const marfa_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int marfa_ctx_create(int device, marfa_ctx **out) {
+    if (!out) return MARFA_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MARFA_E_NODEVICE;
+    if (device < 0 || device >= ndev) return MARFA_E_ARG;
+    marfa_ctx *c = new marfa_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return MARFA_E_HIP;
+    }
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    *out = c;
+    return 0;
+}
+
+int marfa_ctx_destroy(marfa_ctx *c) {
+    if (!c) return MARFA_E_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->d_nu, c->d_S, c->d_gair, c->d_gself, c->d_elow, c->d_nT, c->d_delta, c->d_molIso, c->d_tips};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    marfa_ctx::Buf *bufs[] = {&c->params[0], &c->params[1], &c->params[2], &c->params[3], &c->keys[0], &c->keys[1],
+                              &c->idx[0], &c->idx[1], &c->sorttmp, &c->coarse, &c->lvl, &c->win, &c->out, &c->counts};
+    for (auto *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int marfa_set_tips(marfa_ctx *c, int32_t niso, int32_t ntemp, const float *tips) {
+    if (!c || !tips || niso != 74 || ntemp != 501) return set_err(c, MARFA_E_ARG, "TIPS must be 74 x 501");
+    (void)hipSetDevice(c->device);
+    if (!c->d_tips) HIPCHK(hipMalloc(&c->d_tips, 74 * 501 * 4));
+    HIPCHK(hipMemcpy(c->d_tips, tips, 74 * 501 * 4, hipMemcpyHostToDevice));
+    c->tips_set = true;
+    return 0;
+}
+
+int marfa_set_lines(marfa_ctx *c, int64_t n, const double *nu, const float *S, const float *gair,
+                    const float *gself, const float *elow, const float *nT, const float *delta,
+                    const int32_t *molIso) {
+    if (!c || n < 0 || (n > 0 && (!nu || !S || !gair || !gself || !elow || !nT || !delta || !molIso)))
+        return set_err(c, MARFA_E_ARG, "bad line arrays");
+    if (n >= (int64_t)1 << 31) return set_err(c, MARFA_E_ARG, "at most 2^31-1 lines per context");
+    for (int64_t i = 0; i < n; i++) {
+        const int iso = molIso[i] / 100;
+        if (iso < 1 || iso > 74) return set_err(c, MARFA_E_ARG, "isotopologue outside the TIPS/WISO tables");
+        if (i && !(nu[i] >= nu[i - 1])) return set_err(c, MARFA_E_ARG, "line centres must be ascending");
+    }
+    (void)hipSetDevice(c->device);
+    void **ptrs[] = {(void **)&c->d_nu, (void **)&c->d_S, (void **)&c->d_gair, (void **)&c->d_gself,
+                     (void **)&c->d_elow, (void **)&c->d_nT, (void **)&c->d_delta, (void **)&c->d_molIso};
+    for (void **p : ptrs)
+        if (*p) {
+            HIPCHK(hipFree(*p));
+            *p = nullptr;
+        }
+    const size_t m = (size_t)std::max<int64_t>(n, 1);
+    HIPCHK(hipMalloc(&c->d_nu, m * 8));
+    HIPCHK(hipMalloc(&c->d_S, m * 4));
+    HIPCHK(hipMalloc(&c->d_gair, m * 4));
+    HIPCHK(hipMalloc(&c->d_gself, m * 4));
+    HIPCHK(hipMalloc(&c->d_elow, m * 4));
+    HIPCHK(hipMalloc(&c->d_nT, m * 4));
+    HIPCHK(hipMalloc(&c->d_delta, m * 4));
+    HIPCHK(hipMalloc(&c->d_molIso, m * 4));
+    if (n > 0) {
+        HIPCHK(hipMemcpy(c->d_nu, nu, n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_S, S, n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_gair, gair, n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_gself, gself, n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_elow, elow, n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_nT, nT, n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_delta, delta, n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_molIso, molIso, n * 4, hipMemcpyHostToDevice));
+    }
+    c->h_nu.assign(nu, nu + n);
+    c->n = n;
+    return 0;
+}
+
+int marfa_set_levels(marfa_ctx *c, int32_t nlev, const float *P, const float *T, const float *rho) {
+    if (!c || nlev < 0 || nlev > 200 || (nlev > 0 && (!P || !T || !rho)))
+        return set_err(c, MARFA_E_ARG, "bad levels (at most 200, src/Atmsosphere.f90:37)");
+    c->hP.assign(P, P + nlev);
+    c->hT.assign(T, T + nlev);
+    c->hrho.assign(rho, rho + nlev);
+    return 0;
+}
+
+int marfa_windows(marfa_ctx *c, const marfa_job *job, int32_t max_rec, int32_t *nrec, int64_t *lo,
+                  int64_t *hi, int32_t *recnum, double *origin) {
+    if (!c || !job || !nrec) return set_err(c, MARFA_E_ARG, "null argument");
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    *nrec = w.nrec;
+    if (w.nrec > max_rec) return 0;
+    for (int i = 0; i < w.nrec; i++) {
+        if (lo) lo[i] = w.lo[i];
+        if (hi) hi[i] = w.hi[i];
+        if (recnum) recnum[i] = w.rec[i];
+        if (origin) origin[i] = w.org[i];
+    }
+    return 0;
+}
+
+static int check_job(marfa_ctx *c, const marfa_job *job) {
+    if (!c || !job) return MARFA_E_ARG;
+    if (!c->tips_set || c->n == 0) return set_err(c, MARFA_E_STATE, "lines and TIPS must be set");
+    if (job->shape_id < 0 || job->shape_id > 1 || job->chi_id < 0 || job->chi_id > 3 ||
+        (job->precision != MARFA_FP64 && job->precision != MARFA_FP32K) || !(job->cutoff >= 0.f) ||
+        !(job->end_wv > job->start_wv))
+        return set_err(c, MARFA_E_ARG, "bad job parameters");
+    return 0;
+}
+
+static LevelSet level_set(const marfa_ctx *c, const marfa_job &job) {
+    LevelSet L;
+    L.P = c->hP;
+    L.T = c->hT;
+    for (size_t i = 0; i < c->hP.size(); i++) {
+        // pSelf of app/main.f90:92 in FP64
+        L.pself.push_back((double)c->hrho[i] * 10.0 * (double)MARFA_BOLSI_F * (double)c->hT[i] / (double)MARFA_PATM_F);
+        L.outmul.push_back(job.target == MARFA_VAC ? c->hrho[i] : 1.0f);
+    }
+    return L;
+}
+
+int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *out_host, void *stream,
+                    int32_t async) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
+    (void)hipSetDevice(c->device);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    LevelSet L = level_set(c, *job);
+    const size_t total = (size_t)L.P.size() * w.nrec * NT;
+    float *out = out_dev;
+    if (!out) {
+        if (ensure(c, c->out, total * 4)) return MARFA_E_HIP;
+        out = (float *)c->out.p;
+    }
+    rc = run(c, *job, L, w, out, st, true);
+    if (rc) return rc;
+    if (out_host) {
+        HIPCHK(hipMemcpyAsync(out_host, out, total * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    } else if (!async) {
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return 0;
+}
+
+int marfa_lbl_scheme(marfa_ctx *c, int64_t *line_idx, double cap_wv, double start_dwv, double ext_end_dwv,
+                     float cutoff, float P, float T, float pSelf, int32_t shape_id, int32_t chi_id,
+                     int32_t mol_code, int32_t precision, float *rk) {
+    marfa_job job = {start_dwv, start_dwv + 10.0, cutoff, shape_id, chi_id, mol_code, MARFA_ACS, precision};
+    int rc = check_job(c, &job);
+    if (rc) return rc;
+    if (!line_idx || !rk) return set_err(c, MARFA_E_ARG, "null argument");
+    const int64_t lo = *line_idx - 1;
+    if (lo < 0 || lo >= c->n) return set_err(c, MARFA_E_WINDOW, "line index outside the line list");
+    (void)hipSetDevice(c->device);
+    const auto &nu = c->h_nu;
+    // src/LBL.f90:41-61: read until the unshifted centre reaches extEndDeltaWV, carry the
+    // last line at or below capWV
+    int64_t hi = std::lower_bound(nu.begin() + lo, nu.end(), ext_end_dwv) - nu.begin();
+    int64_t ub = std::upper_bound(nu.begin() + lo, nu.begin() + hi, cap_wv) - nu.begin();
+    if (ub - 1 >= lo) *line_idx = ub;  // 1-based index of the last line <= capWV
+    Windows w;
+    w.nrec = 1;
+    w.lo = {lo};
+    w.hi = {std::max(hi, lo)};
+    w.rec = {(int32_t)((start_dwv + 1.0) / 10.0)};
+    w.org = {start_dwv};
+    if (hi <= lo) {
+        std::fill(rk, rk + NT, 0.f);
+        return 0;
+    }
+    LevelSet L;
+    L.P = {P};
+    L.T = {T};
+    L.pself = {(double)pSelf};
+    L.outmul = {1.0f};
+    if (ensure(c, c->out, (size_t)NT * 4)) return MARFA_E_HIP;
+    rc = run(c, job, L, w, (float *)c->out.p, c->stream, false);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(rk, c->out.p, NT * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (!samples || c->hP.empty()) return set_err(c, MARFA_E_ARG, "levels and output required");
+    (void)hipSetDevice(c->device);
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    LevelSet L = level_set(c, *job);
+    const int nlev = (int)L.P.size();
+    *samples = 0;
+    if (w.nrec == 0) return 0;
+    // shifted centres for every level, then one workgroup per record
+    const int64_t lmin = *std::min_element(w.lo.begin(), w.lo.end());
+    const int64_t lmax = *std::max_element(w.hi.begin(), w.hi.end());
+    const int64_t nl = std::max<int64_t>(lmax - lmin, 1);
+    hipStream_t st = c->stream;
+    int64_t total = 0;
+    for (int l = 0; l < nlev; l++) {
+        LevelSet one;
+        one.P = {L.P[l]};
+        one.T = {L.T[l]};
+        one.pself = {L.pself[l]};
+        one.outmul = {1.f};
+        if (ensure(c, c->lvl, 64 + 3 * 4 + 2 * 8)) return MARFA_E_HIP;
+        for (int q = 0; q < 4; q++)
+            if (ensure(c, c->params[q], (size_t)nl * 8)) return MARFA_E_HIP;
+        if (ensure(c, c->keys[0], (size_t)nl * 8) || ensure(c, c->idx[0], (size_t)nl * 4)) return MARFA_E_HIP;
+        if (ensure(c, c->win, (size_t)w.nrec * 24 + 64)) return MARFA_E_HIP;
+        float *dP = (float *)c->lvl.p, *dT = dP + 1;
+        double *dps = (double *)(((uintptr_t)(dP + 3) + 15) & ~(uintptr_t)15);
+        HIPCHK(hipMemcpyAsync(dP, &one.P[0], 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dT, &one.T[0], 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dps, &one.pself[0], 8, hipMemcpyHostToDevice, st));
+        int64_t *dlo = (int64_t *)c->win.p, *dhi = dlo + w.nrec;
+        double *dorg = (double *)(dhi + w.nrec);
+        HIPCHK(hipMemcpyAsync(dlo, w.lo.data(), w.nrec * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dhi, w.hi.data(), w.nrec * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dorg, w.org.data(), w.nrec * 8, hipMemcpyHostToDevice, st));
+        SpecArgs sa = {};
+        sa.nlev = 1;
+        sa.lmin = lmin;
+        sa.nl = nl;
+        sa.nu = c->d_nu;
+        sa.S = c->d_S;
+        sa.gair = c->d_gair;
+        sa.gself = c->d_gself;
+        sa.elow = c->d_elow;
+        sa.nT = c->d_nT;
+        sa.delta = c->d_delta;
+        sa.molIso = c->d_molIso;
+        sa.P = dP;
+        sa.T = dT;
+        sa.pself = dps;
+        sa.tips = c->d_tips;
+        sa.nus = (double *)c->params[0].p;
+        sa.gD = (double *)c->params[1].p;
+        sa.gL = (double *)c->params[2].p;
+        sa.ST = (double *)c->params[3].p;
+        sa.keys = (uint64_t *)c->keys[0].p;
+        sa.idx = (int32_t *)c->idx[0].p;
+        hipLaunchKernelGGL(spectroscopy_kernel, dim3((unsigned)((nl + BLOCK - 1) / BLOCK), 1), dim3(BLOCK), 0, st, sa);
+        HIPCHK(hipGetLastError());
+        KArgs ka = {};
+        ka.nlev = 1;
+        ka.nrec = w.nrec;
+        ka.lmin = lmin;
+        ka.nl = nl;
+        ka.nus = sa.nus;
+        ka.lo = dlo;
+        ka.hi = dhi;
+        ka.org = dorg;
+        ka.cutoff = job->cutoff;
+        if (ensure(c, c->counts, (size_t)w.nrec * 8 + 64)) return MARFA_E_HIP;
+        hipLaunchKernelGGL(count_kernel, dim3(w.nrec), dim3(BLOCK), 0, st, ka, (unsigned long long *)c->counts.p);
+        HIPCHK(hipGetLastError());
+        std::vector<unsigned long long> h(w.nrec);
+        HIPCHK(hipMemcpyAsync(h.data(), c->counts.p, w.nrec * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (auto x : h) total += (int64_t)x;
+    }
+    *samples = total;
+    return 0;
+}
+
+int marfa_grid_sums(marfa_ctx *c, const marfa_job *job, int32_t level, int32_t rec, double *grids,
+                    float *record) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (!grids || level < 0 || level >= (int)c->hP.size()) return set_err(c, MARFA_E_ARG, "bad level");
+    (void)hipSetDevice(c->device);
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    if (rec < 0 || rec >= w.nrec) return set_err(c, MARFA_E_ARG, "bad record");
+    Windows one;
+    one.nrec = 1;
+    one.lo = {w.lo[rec]};
+    one.hi = {w.hi[rec]};
+    one.rec = {w.rec[rec]};
+    one.org = {w.org[rec]};
+    LevelSet all = level_set(c, *job), L;
+    L.P = {all.P[level]};
+    L.T = {all.T[level]};
+    L.pself = {all.pself[level]};
+    L.outmul = {all.outmul[level]};
+    const size_t gbytes = (size_t)(30690 + NT) * 8;
+    if (ensure(c, c->out, (size_t)NT * 4 + gbytes + 64)) return MARFA_E_HIP;
+    double *dg = (double *)(((uintptr_t)((float *)c->out.p + NT) + 15) & ~(uintptr_t)15);
+    HIPCHK(hipMemsetAsync(dg, 0, gbytes, c->stream));
+    c->dbg_grids = dg;
+    rc = run(c, *job, L, one, (float *)c->out.p, c->stream, false);
+    c->dbg_grids = nullptr;
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(grids, dg, gbytes, hipMemcpyDeviceToHost, c->stream));
+    if (record) HIPCHK(hipMemcpyAsync(record, c->out.p, NT * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int marfa_last_timings(const marfa_ctx *c, float *ms5) {
+    if (!c || !ms5) return MARFA_E_ARG;
+    memcpy(ms5, c->timings, sizeof c->timings);
+    return 0;
+}
+
+}  // extern "C"

@@ -540,7 +540,7 @@ int mo_record(int precision, const mo_lines *Ls, const mo_job *J, const float *t
             deposit_line_r32(g, rk, a, s[1], J->cutoff, &p, &c);
         }
         if (grids_out)
-            for (int i = 0; i < MO_NGRID; i++) grids_out[i] = g[i];
+            for (int i = 0; i < MO_NGRID + MO_NT; i++) grids_out[i] = g[i];
         cascade_r32(g, rk);
         for (int i = 0; i < MO_NT; i++) rk_out[i] = rk[i];
         free(g);
@@ -554,7 +554,7 @@ int mo_record(int precision, const mo_lines *Ls, const mo_job *J, const float *t
             f64_line p = {s[2], s[3], s[5], J->shape_id, J->chi_id, J->mol_code, (double)T};
             deposit_line_f64(g, rk, a, s[1], J->cutoff, &p, &c);
         }
-        if (grids_out) memcpy(grids_out, g, sizeof(double) * MO_NGRID);
+        if (grids_out) memcpy(grids_out, g, sizeof(double) * (MO_NGRID + MO_NT));
         cascade_f64(g, rk);
         memcpy(rk_out, rk, sizeof(double) * MO_NT);
         free(g);

@@ -60,8 +60,8 @@ int mo_windows(const mo_lines *L, const mo_job *J, int max_rec, int64_t *lo, int
                int32_t *recnum, double *origin);
 
 /* One record (one level, one subinterval): reset, LBL line loop, cascade.
- * rk_out[20481] receives RK (ACS); grids_out (optional, MO_NGRID doubles, level-major
- * P|M|L blocks) receives the grid sums before the cascade. */
+ * rk_out[20481] receives RK (ACS); grids_out (optional, MO_NGRID + MO_NT doubles:
+ * level-major P|M|L blocks, then RK) receives the sums before the cascade. */
 int mo_record(int precision, const mo_lines *L, const mo_job *J, const float *tips,
               float P, float T, float rho, double a, int64_t lo, int64_t hi,
               double *rk_out, double *grids_out, mo_counts *counts);

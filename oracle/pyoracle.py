@@ -130,7 +130,7 @@ class OracleJob:
 
     def record(self, precision, lev, r, grids=False):
         rk = np.zeros(NT, np.float64)
-        g = np.zeros(NGRID, np.float64) if grids else None
+        g = np.zeros(NGRID + NT, np.float64) if grids else None
         cnt = Counts()
         lib().mo_record(precision, C.byref(self.lines), C.byref(self.job), _ptr(self.tips),
                         float(self.P[lev]), float(self.T[lev]), float(self.rho[lev]),

@@ -1,0 +1,102 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+FP64 form: grid sums and records within 1e-11 relative of the FP64 oracle (same
+selection, different summation order).  FP32K form: records within 1e-4 relative of the
+FP64 oracle.  Selection is checked node by node: the set of grid nodes that receive any
+sample is identical.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from marfa_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+NGRID = 30690
+LEVEL_OFF = [30 * ((1 << k) - 1) for k in range(11)]
+
+
+@pytest.fixture(scope="module")
+def tips():
+    return synth.load_tips()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from marfa_amd import lbl
+    c = lbl.LBLContext(0)
+    yield c
+    c.close()
+
+
+def small_job(seed=7, n=2000, lo=3985.0, hi=4045.0, a=4000.0, b=4030.0, cut=25.0, nlev=3,
+              chi=0, target=0, shape=0, p0=90.0, p1=1e-3):
+    return synth.Job(synth.uniform_lines(n, lo, hi, seed), synth.log_levels(nlev, p0, p1, 735.0, 170.0, 0.965),
+                     a, b, cut, chi_id=chi, target=target, shape_id=shape)
+
+
+def rel_err(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), 1e-300)
+    return np.abs(x - ref) / den
+
+
+def describe_grid_diff(g, ref, tol):
+    bad = np.nonzero(rel_err(g, ref) > tol)[0]
+    out = []
+    for e in bad[:12]:
+        k = max(i for i in range(10) if LEVEL_OFF[i] <= e)
+        o = e - LEVEL_OFF[k]
+        ntk = 10 << k
+        out.append(f"level {k} role {'PML'[o // ntk]} cell {o % ntk + 1}: gpu {g[e]:.6e} ref {ref[e]:.6e}")
+    return f"{bad.size} nodes differ; " + "; ".join(out)
+
+
+@pytest.mark.parametrize("cfg", ["plain", "wide", "chi2", "lowp"])
+def test_grid_sums_fp64(ctx, tips, cfg):
+    from marfa_amd import lbl
+    job = {"plain": small_job(),
+           "wide": small_job(seed=9, n=6000, lo=3960.0, hi=4080.0, cut=25.0),
+           "chi2": small_job(seed=10, n=3000, lo=3900.0, hi=4150.0, cut=125.0, chi=2),
+           "lowp": small_job(seed=11, n=2500, p0=1e-2, p1=1e-4)}[cfg]
+    ctx.load_job(job, tips)
+    oj = po.OracleJob(job, tips)
+    spec = lbl.JobSpec.from_job(job, lbl.FP64)
+    for lev in range(len(job.atm)):
+        for r in range(oj.nrec):
+            print(f"case {cfg} level {lev} record {r}", flush=True)
+            rk_ref, g_ref, _ = oj.record(po.F64, lev, r, grids=True)
+            g, rkd, rec = ctx.grid_sums(spec, lev, r)
+            # selection: the same nodes receive samples
+            assert np.array_equal(g != 0, g_ref[:NGRID] != 0), describe_grid_diff(g, g_ref[:NGRID], 0)
+            assert np.array_equal(rkd != 0, g_ref[NGRID:] != 0), "fine-point selection differs"
+            assert rel_err(g, g_ref[:NGRID]).max() < 1e-11, describe_grid_diff(g, g_ref[:NGRID], 1e-11)
+            assert rel_err(rkd, g_ref[NGRID:]).max() < 1e-11
+            assert rel_err(rec, rk_ref).max() < 1e-6  # record is FP32
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-4)])
+def test_pt_tables_config1(ctx, tips, prec, tol):
+    from marfa_amd import lbl
+    job = synth.config1()
+    ctx.load_job(job, tips)
+    oj = po.OracleJob(job, tips)
+    ref, cnt = oj.pt_tables(po.F64)
+    out = ctx.pt_tables(lbl.JobSpec.from_job(job, prec))
+    err = rel_err(out, ref)
+    print(f"config1 precision {prec}: max rel {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e}", flush=True)
+    assert err.max() < tol
+    ns = ctx.count_samples(lbl.JobSpec.from_job(job, prec))
+    assert ns == cnt.samples, (ns, cnt.samples)
+
+
+def test_determinism(ctx, tips):
+    from marfa_amd import lbl
+    job = small_job(seed=21)
+    ctx.load_job(job, tips)
+    for prec in (lbl.FP64, lbl.FP32K):
+        a = ctx.pt_tables(lbl.JobSpec.from_job(job, prec))
+        b = ctx.pt_tables(lbl.JobSpec.from_job(job, prec))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
