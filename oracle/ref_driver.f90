@@ -10,7 +10,7 @@
 ! which tests/golden/gen_golden.py checks.
 !
 ! usage: ref_driver startWV endWV dbFile cutOff ACS|VAC atmFile chiId molCode outDir
-!        [maxLevels]
+!        [maxLevels [shapeId]]      shapeId 0 voigt (default), 1 lorentzShape
 ! TIPS is read from data/TIPS/TIPS.dat relative to the working directory, as the
 ! reference does (src/EnvironmentSetup.f90:238).
 program ref_driver
@@ -27,7 +27,7 @@ program ref_driver
     character(len=400) :: arg, dbFile, atmFile, outDir
     character(len=8) :: target
     character(len=3) :: levelLabel
-    integer :: chiId, maxLevels, lvl, recNum, startIdx, lineIdx
+    integer :: chiId, maxLevels, lvl, recNum, startIdx, lineIdx, shapeId
     real(kind=DP) :: startLineWV, capWV, t0, t1
     logical :: first
 
@@ -41,12 +41,17 @@ program ref_driver
     call get_command_argument(8, arg); read(arg, *) moleculeIntCode
     call get_command_argument(9, outDir)
     maxLevels = 1000000
+    shapeId = 0
     if (command_argument_count() >= 10) then
         call get_command_argument(10, arg); read(arg, *) maxLevels
+    end if
+    if (command_argument_count() >= 11) then
+        call get_command_argument(11, arg); read(arg, *) shapeId
     end if
 
     call readAtmosphere(trim(atmFile))
     call setShapeFunction()
+    if (shapeId == 1) shapeFuncPtr => lorentzShape
     call setChiFactorFunction()
     select case (chiId)
     case (1)

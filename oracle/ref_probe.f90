@@ -26,6 +26,8 @@
 !          cascadeInterpolation (src/Grids.f90:104-139) and prints RK(1..20481)
 !   wiso
 !       -> bit patterns of WISO(1..124) (src/MolarMasses.f90:140-146)
+!   tips
+!       -> bit patterns of TIPS(74,501) as read by the reference's reader, row by row
 !   quit
 program ref_probe
     use Constants
@@ -62,6 +64,10 @@ program ref_probe
             write(*, '(7(Z8.8,1X))') transfer(PI, 0), transfer(sqln2, 0), transfer(C2, 0), &
                 transfer(dopplerCONST, 0), transfer(sqrt(pi), 0), transfer(sqrt(1.4), 0), &
                 transfer(log(2.), 0)
+        case ('tips')
+            do i = 1, size(TIPS, 1)
+                write(*, '(501(Z8.8,1X))') (transfer(TIPS(i, chiId), 0), chiId = 1, size(TIPS, 2))
+            end do
         case ('wiso')
             do i = 1, size(WISO)
                 write(*, '(Z8.8)') transfer(WISO(i), 0)
