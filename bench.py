@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""PT-table generation throughput of the MI355X LBL core on one GPU.
+
+Workload (default): 2,000,000 synthetic CO2-like lines over 500-10,000 cm-1 in 20
+Gaussian bands, 60 levels (P 90 -> 1e-4 atm, T 735 -> 170 K), cutoff 125 cm-1, Pollack
+chi factor, FP32 samples with Kahan-compensated sums and FP64 offsets.  One step = every
+PT-table record of every level (60 x 950 records x 20,481 points), inputs resident in HBM,
+records left in HBM.
+
+Printed JSON line: value = PT-table points/s; roofline of the dominant kernel (the
+level-0 deposit) from HIP-event timings of the timed steps; accuracy vs the FP64 oracle on
+sampled records; cpu_baseline = the reference's own Fortran LBL modules (built from its
+sources into oracle/_ref) timed on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_SAMPLE = 12          # nominal FLOPs per profile sample (Lorentz-dominated mix)
+FLOP_PER_RECORD_CASCADE = 153440
+PEAK = {"fp32": 157.3, "fp64": 78.6}  # TFLOP/s, MI355X vector peaks
+HBM_PEAK = 8000.0             # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def accuracy(job, tips, gpu_records, picks):
+    """max / rms relative error of GPU records against the FP64 oracle (gridded)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    oj = po.OracleJob(job, tips)
+    errs, maxe = [], 0.0
+    for lev, r in picks:
+        ref, _ = oj.record(po.F64, lev, r)
+        g = gpu_records[(lev, r)].astype(np.float64)
+        m = np.abs(ref) >= np.finfo(np.float32).tiny
+        e = np.abs(g[m] - ref[m]) / np.abs(ref[m])
+        errs.append(e)
+        maxe = max(maxe, float(e.max()))
+    e = np.concatenate(errs)
+    return {"max_rel_err": maxe, "rms_rel_err": float(np.sqrt(np.mean(e ** 2))),
+            "records_checked": len(picks), "vs": "FP64 gridded oracle (oracle/marfa_oracle.c, MO_F64)"}
+
+
+def cpu_baseline(job, ctx_counts_full, records=200, level=0):
+    """Time the reference's LBL modules (oracle/_ref/ref_driver, one thread) on `records`
+    subintervals of one level, and scale to the full workload by profile samples."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refrun
+    from marfa_amd import synth, lbl
+    if not refrun.available():
+        return None
+    a = float(job.start_wv)
+    b = min(float(job.end_wv), a + 10.0 * records)
+    atm = synth.Atmosphere(P=job.atm.P[level:level + 1], T=job.atm.T[level:level + 1],
+                           rho=job.atm.rho[level:level + 1], h=job.atm.h[level:level + 1])
+    sub = synth.Job(job.lines, atm, a, b, job.cutoff, mol_code=job.mol_code, chi_id=job.chi_id,
+                    shape_id=job.shape_id, target=job.target)
+    recs, wall, out = refrun.run(sub, use_driver=True)
+    lbl_s = None
+    for line in out.splitlines():
+        if line.startswith("LBL_SECONDS"):
+            lbl_s = float(line.split()[1])
+    # samples of the sample (GPU counter, pinned against the oracle's counters in tests)
+    ctx = lbl.LBLContext(0)
+    ctx.load_job(sub, synth.load_tips())
+    s_sub = ctx.count_samples(lbl.JobSpec.from_job(sub))
+    ctx.close()
+    pts_sub = recs.shape[0] * recs.shape[1] * lbl.NT
+    t = lbl_s if lbl_s else wall
+    samples_full, points_full = ctx_counts_full
+    sps = s_sub / t
+    return {"value": sps * points_full / samples_full, "unit": "pts/s", "cores": 1, "kind": "reference",
+            "sample": (f"reference LBL modules (src/*.f90, amdflang -O2, chi=pollack) via oracle/ref_driver: "
+                       f"level {level + 1}, {recs.shape[1]} records {a:.0f}-{b:.0f} cm-1, {t:.1f} s in the "
+                       f"level/subinterval loops ({wall:.1f} s wall); {sps:.3e} samples/s scaled to the full "
+                       f"workload by its samples per point"),
+            "sample_pts_per_s": pts_sub / t, "samples_per_s": sps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp32k", "fp64"], default="fp32k")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 drop-in timing")
+    ap.add_argument("--cpu-records", type=int, default=150)
+    ap.add_argument("--nlines", type=int, default=2_000_000)
+    ap.add_argument("--nlev", type=int, default=60)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(levels and subintervals shard with no communication; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from marfa_amd import synth, lbl
+
+    t0 = time.time()
+    job = synth.config2(nlines=args.nlines, nlev=args.nlev)
+    tips = synth.load_tips()
+    log(f"[bench] workload generated in {time.time() - t0:.1f}s: {len(job.lines)} lines, {len(job.atm)} levels")
+    ctx = lbl.LBLContext(0)
+    ctx.load_job(job, tips)
+    prec = lbl.FP32K if args.precision == "fp32k" else lbl.FP64
+    spec = lbl.JobSpec.from_job(job, prec)
+    lo, hi, recnum, org = ctx.windows(spec)
+    nrec = lo.shape[0]
+    nlev = len(job.atm)
+    points = nlev * nrec * lbl.NT
+    samples, coarse_evals = ctx.count_samples(spec, coarse=True)
+    log(f"[bench] {nrec} records/level, {points:.4e} points, {samples:.4e} samples ({coarse_evals:.4e} level-0)")
+
+    out = torch.empty((nlev, nrec, lbl.NT), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    for w in range(args.warmup):
+        ctx.pt_tables_device(spec, out.data_ptr(), stream)
+        log(f"[bench] warmup {w} done")
+    torch.cuda.synchronize()
+    phase = np.zeros(5)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.pt_tables_device(spec, out.data_ptr(), stream)
+        phase += np.array(list(ctx.timings().values()))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    phase /= args.steps
+    log(f"[bench] {args.steps} steps: {dt * 1e3:.1f} ms/step; phases (ms) {dict(zip(('spec', 'sort', 'coarse', 'fine', 'total'), phase.round(2)))}")
+    value = points / dt
+
+    # roofline of the dominant kernel (level-0 deposit): algorithmic FLOPs / its event time
+    coarse_s = phase[2] / 1e3
+    dt_name = "fp32" if prec == lbl.FP32K else "fp64"
+    achieved = FLOP_PER_SAMPLE * coarse_evals / coarse_s / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_coarse_kernel_{dt_name}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roof = {"bound": "valu", "kernel": "coarse_kernel", "achieved": achieved, "peak": PEAK[dt_name],
+            "unit": "TFLOP/s", "frac": achieved / PEAK[dt_name], "traffic": traffic,
+            "flop_model": f"{FLOP_PER_SAMPLE} FLOP x {coarse_evals} level-0 samples per launch"}
+    whole = (FLOP_PER_SAMPLE * samples + FLOP_PER_RECORD_CASCADE * nlev * nrec) / dt / 1e12
+
+    result = {
+        "metric": "PT-table spectral pts/s at <=1e-4 rel err vs FP64; fwd+adjoint ms per level",
+        "value": value, "unit": "pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == lbl.FP32K else "f64", "data": "synthetic (seeded, config 2 laws)",
+        "config": {"workload": "2,000,000 lines 500-10000 cm-1 (20 Gaussian bands), 60 levels P 90->1e-4 atm "
+                               "T 735->170 K, cutoff 125 cm-1, Pollack chi, Voigt",
+                   "lines": len(job.lines), "levels": nlev, "records_per_level": nrec, "points": points,
+                   "samples": samples, "form": "FP32+Kahan, FP64 offsets" if prec == lbl.FP32K else "FP64 plain"},
+        "samples_per_s": samples / dt,
+        "whole_path_tflops": whole,
+        "phases_ms": dict(zip(("spectroscopy", "sort", "coarse", "fine", "total"), phase.tolist())),
+        "roofline": roof,
+    }
+
+    if not args.no_fp64 and prec == lbl.FP32K:
+        spec64 = lbl.JobSpec.from_job(job, lbl.FP64)
+        ctx.pt_tables_device(spec64, out.data_ptr(), stream)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.pt_tables_device(spec64, out.data_ptr(), stream)
+        torch.cuda.synchronize()
+        d64 = time.perf_counter() - t1
+        result["fp64_dropin"] = {"ms_per_step": d64 * 1e3, "pts_per_s": points / d64,
+                                 "phases_ms": ctx.timings()}
+        log(f"[bench] FP64 drop-in: {d64 * 1e3:.1f} ms/step")
+        ctx.pt_tables_device(spec, out.data_ptr(), stream)  # restore FP32 records for accuracy
+        torch.cuda.synchronize()
+
+    if not args.no_accuracy:
+        rng = np.random.Generator(np.random.PCG64(5))
+        picks = [(int(l), int(r)) for l, r in zip(rng.integers(0, nlev, 12), rng.integers(0, nrec, 12))]
+        recs = {(l, r): out[l, r].cpu().numpy() for l, r in picks}
+        t2 = time.time()
+        result["accuracy"] = accuracy(job, tips, recs, picks)
+        log(f"[bench] accuracy {result['accuracy']} ({time.time() - t2:.1f}s)")
+
+    if not args.no_cpu_baseline:
+        t3 = time.time()
+        result["cpu_baseline"] = cpu_baseline(job, (samples, points), records=args.cpu_records)
+        log(f"[bench] cpu baseline {result['cpu_baseline']} ({time.time() - t3:.1f}s)")
+        if result["cpu_baseline"]:
+            result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -118,9 +118,11 @@ int marfa_lbl_scheme(marfa_ctx *ctx, int64_t *line_idx, double cap_wv, double st
 int marfa_pt_tables(marfa_ctx *ctx, const marfa_job *job, float *out_dev, float *out_host,
                     void *stream, int32_t async);
 
-/* exact count of distinct profile evaluations (samples) of the job's grid geometry,
- * summed over levels and subintervals (computed on the device). */
-int marfa_count_samples(marfa_ctx *ctx, const marfa_job *job, int64_t *samples);
+/* Work counts of the job, summed over levels and subintervals (computed on the device):
+ * samples[0] = exact count of distinct profile evaluations (samples) of the reference's
+ * grid geometry (21 + 2m per wing line, the walks and core points of a centre line);
+ * samples[1] = profile evaluations made by the level-0 deposit kernel. */
+int marfa_count_samples(marfa_ctx *ctx, const marfa_job *job, int64_t samples[2]);
 
 /* Pre-cascade grid sums of one record (level, subinterval index), in the reference's
  * layout: for grid k = 0..9 the arrays RKkP, RKk, RKkL (src/Grids.f90:34-45) of 10*2^k

@@ -188,6 +188,46 @@ __global__ __launch_bounds__(BLOCK) void smax_kernel(const double *ST, int64_t n
 // Bit q of a mask marks position q*0.5 cm-1 of the subinterval (q = 0..20).
 constexpr uint32_t EVEN = 0x155555u, ODD = 0x0AAAAAu;
 
+// Roles of one line at the 21 positions of grid 0 (bit q: position q/2 cm-1): P, mid and
+// L edges of cells 1..10.  Returns false when the line deposits nothing in the record.
+__device__ inline bool level0_masks(double nus, double org, double cut, uint32_t &mP, uint32_t &mM,
+                                    uint32_t &mL) {
+    const double u = nus - org;
+    mP = mM = mL = 0;
+    if (nus < org) {                 // leftLBL, src/LineGridCalc.f90:21-33: cells 1..10 (d >= 1)
+        const double d = -u;         // or 2..10 (finer grids cover [0, 1])
+        if (d > cut) return false;
+        if (d >= 1.0) { mP = EVEN & 0x7FFFFu; mM = ODD; mL = EVEN & ~1u; }
+        else { mP = EVEN & 0x7FFFCu; mM = ODD & ~2u; mL = EVEN & ~5u; }
+        return true;
+    }
+    if (nus > org + 10.0) {          // rightLBL, :508-518: cells 10..1 or 9..1
+        const double dr = u - 10.0;
+        if (dr >= cut) return false;
+        if (dr >= 1.0) { mP = EVEN & 0x7FFFFu; mM = ODD; mL = EVEN & ~1u; }
+        else { mP = EVEN & 0x1FFFFu; mM = ODD & 0x3FFFFu; mL = EVEN & 0x7FFFCu; }
+        return true;
+    }
+    if (!(u < 10.0)) return false;   // centerLBL :192 (a line at the right edge adds nothing)
+    // centerLBL level-0 walks (:201-214 from the left edge, :343-358 from the right edge):
+    // cells 1..e0 and 11-e0'..10 with e0 = floor(u) - 1, e0' = floor(10 - u) - 1
+    const int e0 = u >= 2.0 ? (int)floor(u) - 1 : 0;
+    const double v = 10.0 - u;
+    const int e0r = v >= 2.0 ? (int)floor(v) - 1 : 0;
+    if (e0 > 0) {
+        mP = EVEN & ((1u << (2 * e0 - 1)) - 1u);
+        mM = ODD & ((1u << (2 * e0)) - 1u);
+        mL = EVEN & ((1u << (2 * e0 + 1)) - 1u) & ~1u;
+    }
+    if (e0r > 0) {
+        const int qs = 20 - 2 * e0r;
+        mP |= EVEN & ~((1u << qs) - 1u) & 0x7FFFFu;
+        mM |= ODD & ~((1u << qs) - 1u);
+        mL |= EVEN & ~((1u << (qs + 1)) - 1u);
+    }
+    return true;
+}
+
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
     const int nitems = a.nlev * a.nrec;
@@ -210,39 +250,8 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
     for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) {
         const double nus = a.nus[base + i];
         const double u = nus - org;
-        uint32_t mP = 0, mM = 0, mL = 0;
-        bool use = false;
-        if (nus < org) {                 // leftLBL, src/LineGridCalc.f90:21-33
-            const double d = -u;
-            if (!(d > cut)) {
-                use = true;
-                if (d >= 1.0) { mP = EVEN & 0x7FFFFu; mM = ODD; mL = EVEN & ~1u; }
-                else { mP = EVEN & 0x7FFFCu; mM = ODD & ~2u; mL = EVEN & ~5u; }
-            }
-        } else if (nus > top) {          // rightLBL, :508-518
-            const double dr = u - 10.0;
-            if (!(dr >= cut)) {
-                use = true;
-                if (dr >= 1.0) { mP = EVEN & 0x7FFFFu; mM = ODD; mL = EVEN & ~1u; }
-                else { mP = EVEN & 0x1FFFFu; mM = ODD & 0x3FFFFu; mL = EVEN & 0x7FFFCu; }
-            }
-        } else if (u < 10.0) {           // centerLBL level-0 walks, :201-214 and :343-358
-            use = true;
-            const int e0 = u >= 2.0 ? (int)floor(u) - 1 : 0;
-            const double v = 10.0 - u;
-            const int e0r = v >= 2.0 ? (int)floor(v) - 1 : 0;
-            if (e0 > 0) {
-                mP = EVEN & ((1u << (2 * e0 - 1)) - 1u);
-                mM = ODD & ((1u << (2 * e0)) - 1u);
-                mL = EVEN & ((1u << (2 * e0 + 1)) - 1u) & ~1u;
-            }
-            if (e0r > 0) {
-                const int qs = 20 - 2 * e0r;
-                mP |= EVEN & ~((1u << qs) - 1u) & 0x7FFFFu;
-                mM |= ODD & ~((1u << qs) - 1u);
-                mL |= EVEN & ~((1u << (qs + 1)) - 1u);
-            }
-        }
+        uint32_t mP, mM, mL;
+        const bool use = level0_masks(nus, org, cut, mP, mM, mL);
         if (!use) continue;
         const LineShape<R> p = make_line<R>(a.gD[base + i], a.gL[base + i], a.S[base + i] * sc);
         const uint32_t need = mP | mM | mL | 1u;
@@ -618,6 +627,8 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
 
 // ------------------------------------------------------------------ sample counts
 __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
+    // counts[2r] = distinct samples of the reference's geometry in record r,
+    // counts[2r+1] = evaluations made by coarse_kernel for record r
     const int nitems = a.nlev * a.nrec;
     const int r = blockIdx.x;
     if (r >= nitems) return;
@@ -625,9 +636,11 @@ __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long lon
     const double org = a.org[s];
     const double cut = (double)a.cutoff;
     const int64_t base = (int64_t)lev * a.nl - a.lmin;
-    unsigned long long cnt = 0;
+    unsigned long long cnt = 0, c0 = 0;
     for (int64_t i = a.lo[s] + threadIdx.x; i < a.hi[s]; i += BLOCK) {
         const double nus = a.nus[base + i], u = nus - org;
+        uint32_t mP, mM, mL;
+        if (level0_masks(nus, org, cut, mP, mM, mL)) c0 += __builtin_popcount(mP | mM | mL | 1u);
         if (nus < org) {
             const double d = -u;
             if (d > cut) continue;
@@ -659,14 +672,20 @@ __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long lon
             cnt += 2 + 2 * cells + fine;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-    __shared__ unsigned long long red[BLOCK / 64];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    for (int off = 32; off > 0; off >>= 1) {
+        cnt += __shfl_xor(cnt, off);
+        c0 += __shfl_xor(c0, off);
+    }
+    __shared__ unsigned long long red[2][BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = cnt;
+        red[1][threadIdx.x >> 6] = c0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 2) {
         unsigned long long x = 0;
-        for (int w = 0; w < BLOCK / 64; w++) x += red[w];
-        counts[r] = x;
+        for (int w = 0; w < BLOCK / 64; w++) x += red[threadIdx.x][w];
+        counts[2 * r + threadIdx.x] = x;
     }
 }
 
@@ -1137,6 +1156,7 @@ int marfa_lbl_scheme(marfa_ctx *c, int64_t *line_idx, double cap_wv, double star
 }
 
 int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
+    // samples[0]: distinct samples of the reference geometry; samples[1]: coarse_kernel evaluations
     int rc = check_job(c, job);
     if (rc) return rc;
     if (!samples || c->hP.empty()) return set_err(c, MARFA_E_ARG, "levels and output required");
@@ -1145,14 +1165,14 @@ int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
     compute_windows(c->h_nu, *job, w);
     LevelSet L = level_set(c, *job);
     const int nlev = (int)L.P.size();
-    *samples = 0;
+    samples[0] = samples[1] = 0;
     if (w.nrec == 0) return 0;
     // shifted centres for every level, then one workgroup per record
     const int64_t lmin = *std::min_element(w.lo.begin(), w.lo.end());
     const int64_t lmax = *std::max_element(w.hi.begin(), w.hi.end());
     const int64_t nl = std::max<int64_t>(lmax - lmin, 1);
     hipStream_t st = c->stream;
-    int64_t total = 0;
+    int64_t total = 0, total0 = 0;
     for (int l = 0; l < nlev; l++) {
         LevelSet one;
         one.P = {L.P[l]};
@@ -1208,15 +1228,19 @@ int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
         ka.hi = dhi;
         ka.org = dorg;
         ka.cutoff = job->cutoff;
-        if (ensure(c, c->counts, (size_t)w.nrec * 8 + 64)) return MARFA_E_HIP;
+        if (ensure(c, c->counts, (size_t)w.nrec * 16 + 64)) return MARFA_E_HIP;
         hipLaunchKernelGGL(count_kernel, dim3(w.nrec), dim3(BLOCK), 0, st, ka, (unsigned long long *)c->counts.p);
         HIPCHK(hipGetLastError());
-        std::vector<unsigned long long> h(w.nrec);
-        HIPCHK(hipMemcpyAsync(h.data(), c->counts.p, w.nrec * 8, hipMemcpyDeviceToHost, st));
+        std::vector<unsigned long long> h(2 * w.nrec);
+        HIPCHK(hipMemcpyAsync(h.data(), c->counts.p, w.nrec * 16, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        for (auto x : h) total += (int64_t)x;
+        for (int q = 0; q < w.nrec; q++) {
+            total += (int64_t)h[2 * q];
+            total0 += (int64_t)h[2 * q + 1];
+        }
     }
-    *samples = total;
+    samples[0] = total;
+    samples[1] = total0;
     return 0;
 }
 

@@ -197,11 +197,11 @@ class LBLContext:
         self.pt_tables_device(spec, out.data_ptr(), st, sync=True)
         return out
 
-    def count_samples(self, spec: JobSpec) -> int:
-        n = C.c_int64()
+    def count_samples(self, spec: JobSpec, coarse: bool = False):
+        n = np.zeros(2, np.int64)
         j = spec._c()
-        self._check(self.lib.marfa_count_samples(self.h, C.byref(j), C.byref(n)))
-        return int(n.value)
+        self._check(self.lib.marfa_count_samples(self.h, C.byref(j), n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return (int(n[0]), int(n[1])) if coarse else int(n[0])
 
     def grid_sums(self, spec: JobSpec, level: int, rec: int):
         g = np.zeros(30690 + NT, np.float64)
