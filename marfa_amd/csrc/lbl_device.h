@@ -198,7 +198,7 @@ __device__ inline R doppler(R X, const LineShape<R> &p) {
 // Non-Lorentz part of voigt(X): the region is decided from FP64 VX/VY formed exactly as
 // the FP64 form forms them (src/Shapes.f90:49-50), the value in precision R.
 template <typename R>
-__device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const ShapeCtl &c) {
+__device__ inline R voigt_regions(double X64, const LineShape<R> &p, const ShapeCtl &c) {
     const double sq = (double)MARFA_SQLN2_F;
     const double VX = fabs((sq * X64) / p.gD64);
     const double VY = sq * p.gL64 / p.gD64;
@@ -228,8 +228,16 @@ __device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const Sh
     return doppler<R>(X, p);
 }
 
-// Profile value at signed distance X64 (FP64) from the shifted centre.
+// out-of-line copy for kernels where the non-Lorentz regions are rare (level-0 deposits)
 template <typename R>
+__device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const ShapeCtl &c) {
+    return voigt_regions<R>(X64, p, c);
+}
+
+// Profile value at signed distance X64 (FP64) from the shifted centre.  NEAR: the caller
+// samples close to line centres, where the Humlicek-Kuntz regions are the common case,
+// so they are inlined.
+template <typename R, bool NEAR = false>
 __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl &c) {
     const R X = (R)X64;
     if (c.shape_id == 1) return chilor<R>(X, p, c);  // lorentzShape
@@ -237,7 +245,8 @@ __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl
     // fast path: clearly in region 0 (pure Lorentz x chi); margin covers the FP32 error of VX
     const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
     if (ax * p.xs >= margin) return chilor<R>(X, p, c);
-    return voigt_slow<R>(X64, p, c);
+    if constexpr (NEAR) return voigt_regions<R>(X64, p, c);
+    else return voigt_slow<R>(X64, p, c);
 }
 
 // ---------------------------------------------------------------- accumulators

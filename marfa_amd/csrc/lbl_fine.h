@@ -1,0 +1,385 @@
+// lbl_fine.h -- grids 1..9 and the fine points of one 1 cm-1 tile, by gather, then the
+// tile's cascade.  Included by marfa_lbl.hip inside its kernel namespace.
+//
+// Every grid node of the tile is owned by exactly one thread (grids 5..9, fine points) or
+// one wavefront (grids 1..4, whose nodes see hundreds of lines); the owner sums the
+// lines that deposit into it in the order of their shifted centres, so no sum depends on
+// scheduling and no atomics are used.  The lines depositing into a node are found in
+// O(1): a per-level index of the sorted shifted centres in 1/512 cm-1 buckets gives the
+// candidate range, and the reference's walk is inverted in closed form for the exact
+// test.  For a line at u = nu* - a (src/LineGridCalc.f90:191) the left walk of centerLBL
+// (:201-331) covers, on grid k, the cells (2 e_{k-1}, e_k] with e_k = max(0, floor(u 2^k) - 1);
+// the right walk (:343-484) the mirror image in v = 10 - u; the core fine points are
+// [4 e_9 + 2, floor(u/H) + 1] (:332-339) and [floor(u/H) + 2, NT - 4 e'_9] (:485-490).
+// leftLBL / rightLBL deposit into cells 1, 2 (NT_k, NT_k - 1) of grids k <= m, where
+// m = #{k : d < H_k}, and into fine points 2..5 (NT-3..NT) when m = 10.
+#pragma once
+
+constexpr double BUCKETS_PER_CM = 512.0;
+
+__device__ inline int bucket_of(double nus, double base, int NB) {
+    const double x = (nus - base) * BUCKETS_PER_CM;
+    if (!(x >= 0.0)) return 0;
+    if (x >= (double)(NB - 1)) return NB - 1;
+    return (int)x;
+}
+
+// bst[b] = first sorted line whose bucket is >= b, b = 0..NB (each entry written once)
+__global__ __launch_bounds__(BLOCK) void bucket_kernel(const uint64_t *skey, int64_t nl, double base, int NB,
+                                                       int32_t *bst) {
+    const int lev = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j > nl) return;
+    const uint64_t *k = skey + (int64_t)lev * nl;
+    int32_t *o = bst + (int64_t)lev * (NB + 1);
+    const int bprev = j == 0 ? -1 : bucket_of(double_of(k[j - 1]), base, NB);
+    const int bcur = j == nl ? NB : bucket_of(double_of(k[j]), base, NB);
+    for (int b = bprev + 1; b <= bcur; b++) o[b] = (int32_t)j;
+}
+
+// profile parameters of every line, in sorted order, for the fine gathers
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void shape_kernel(const int32_t *sidx, int64_t nl, int64_t lmin,
+                                                      const double *gD, const double *gL, const double *S,
+                                                      const double *sscale, int lev0, LineShape<R> *shp) {
+    const int lev = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nl) return;
+    const int64_t o = (int64_t)lev * nl;
+    const int64_t i = o + (sidx[o + j] - lmin);
+    shp[o + j] = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
+}
+
+template <typename R>
+struct FineCtx {
+    const uint64_t *skey;
+    const int32_t *sidx;
+    const LineShape<R> *shp;
+    const int32_t *bst;
+    double org, top, base, cut;
+    int NB;
+    int64_t wlo, whi;
+    ShapeCtl ctl;
+};
+
+// sorted candidate range [j0, j1) holding every line with u = nu* - org in [u0, u1)
+template <typename R>
+__device__ inline void cand(const FineCtx<R> &f, double u0, double u1, int &j0, int &j1) {
+    const double x0 = (f.org + u0 - f.base) * BUCKETS_PER_CM;
+    const double x1 = (f.org + u1 - f.base) * BUCKETS_PER_CM;
+    const int b0 = (int)fmin(fmax(floor(x0) - 1.0, 0.0), (double)f.NB);
+    const int b1 = (int)fmin(fmax(floor(x1) + 2.0, 0.0), (double)f.NB);
+    j0 = f.bst[b0];
+    j1 = b1 > b0 ? f.bst[b1] : j0;
+}
+
+// 0: no deposit beyond grid 0; 1: left wing with m >= 1; 2: centre line; 3: right wing
+// with m >= 1.  u = nu* - org as src/LineGridCalc.f90 forms it.
+template <typename R>
+__device__ inline int classify(const FineCtx<R> &f, int j, double &u) {
+    const double nus = double_of(f.skey[j]);
+    u = nus - f.org;
+    const int64_t li = f.sidx[j];
+    if (li < f.wlo || li >= f.whi) return 0;
+    if (nus < f.org) {
+        const double d = -u;
+        return (!(d > f.cut) && d < 1.0) ? 1 : 0;
+    }
+    if (nus > f.top) {
+        const double dr = u - 10.0;
+        return (!(dr >= f.cut) && dr < 1.0) ? 3 : 0;
+    }
+    return u < 10.0 ? 2 : 0;
+}
+
+// Sums at one position node of grid k: global position q in steps of H_k/2.  Even q is the
+// right edge (L) of cell q/2 -> aL and the left edge (P) of cell q/2+1 -> aP; odd q is the
+// midpoint of cell (q+1)/2 -> aL.  needL / needP: the roles this tile owns.
+template <typename R>
+__device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool needL, bool needP,
+                                         int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+    const int NTk = 10 << k;
+    const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
+    const bool odd = q & 1;
+    // centre lines, left walk: floor(u 2^k) in [nlo, nhi]
+    {
+        const int nlo = max(2, ((q + 1) >> 1) + 1), nhi = 2 * (q >> 2) + 3;
+        const double ulo = nlo * Hk, uhi = (nhi + 1) * Hk;
+        int j0, j1;
+        cand(f, ulo, uhi, j0, j1);
+        for (int j = j0 + jfirst; j < j1; j += jstep) {
+            double u;
+            if (classify(f, j, u) != 2 || u < ulo || !(u < uhi)) continue;
+            const int n = (int)floor(__builtin_ldexp(u, k));
+            const int lo_q = 4 * max(0, (n >> 1) - 1), hi_q = 2 * (n - 1);
+            if (q < lo_q || q > hi_q) continue;
+            const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
+            if (!toL && !toP) continue;
+            const R val = shape_eval<R, true>(u - xq, f.shp[j], f.ctl);
+            if (toL) aL.add(val);
+            if (toP) aP.add(val);
+        }
+    }
+    // centre lines, right walk: floor(v 2^k) in [nlo, nhi] for q' = 2 NT_k - q, v = 10 - u
+    {
+        const int qp = 2 * NTk - q;
+        const int nlo = max(2, ((qp + 1) >> 1) + 1), nhi = 2 * (qp >> 2) + 3;
+        const double vlo = nlo * Hk, vhi = (nhi + 1) * Hk;
+        int j0, j1;
+        cand(f, 10.0 - vhi - 1e-9, 10.0 - vlo + 1e-9, j0, j1);
+        for (int j = j0 + jfirst; j < j1; j += jstep) {
+            double u;
+            if (classify(f, j, u) != 2) continue;
+            const double v = 10.0 - u;
+            if (v < vlo || !(v < vhi)) continue;
+            const int n = (int)floor(__builtin_ldexp(v, k));
+            const int lo_q = 2 * NTk - 2 * (n - 1), hi_q = 2 * NTk - 4 * max(0, (n >> 1) - 1);
+            if (q < lo_q || q > hi_q) continue;
+            const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
+            if (!toL && !toP) continue;
+            const R val = shape_eval<R, true>(v - qp * hq, f.shp[j], f.ctl);
+            if (toL) aL.add(val);
+            if (toP) aP.add(val);
+        }
+    }
+    // wing lines closer than H_{k-1} to the left edge: cells 1 (m == k) and 2 (m >= k)
+    if (q <= 4) {
+        const double hkm1 = hk(k - 1);
+        int j0, j1;
+        cand(f, -hkm1, 0.0, j0, j1);
+        for (int j = j0 + jfirst; j < j1; j += jstep) {
+            double u;
+            if (classify(f, j, u) != 1) continue;
+            const double d = -u;
+            if (!(d < hkm1)) continue;
+            const bool eq = d >= Hk;
+            bool toL, toP;
+            if (q == 0) { toL = false; toP = eq; }
+            else if (q == 1) { toL = eq; toP = false; }
+            else if (q == 2) { toL = eq; toP = true; }
+            else { toL = true; toP = false; }
+            toL = toL && needL;
+            toP = toP && needP;
+            if (!toL && !toP) continue;
+            const R val = shape_eval<R, true>(u - xq, f.shp[j], f.ctl);
+            if (toL) aL.add(val);
+            if (toP) aP.add(val);
+        }
+    }
+    // wing lines closer than H_{k-1} to the right edge: cells NT_k (m == k), NT_k - 1
+    if (2 * NTk - q <= 4) {
+        const int qp = 2 * NTk - q;
+        const double hkm1 = hk(k - 1);
+        int j0, j1;
+        cand(f, 10.0 - 1e-9, 10.0 + hkm1 + 1e-9, j0, j1);
+        for (int j = j0 + jfirst; j < j1; j += jstep) {
+            double u;
+            if (classify(f, j, u) != 3) continue;
+            const double dr = u - 10.0;
+            if (!(dr < hkm1)) continue;
+            const bool eq = dr >= Hk;
+            bool toL, toP;
+            if (qp == 0) { toL = eq; toP = false; }
+            else if (qp == 1) { toL = eq; toP = false; }
+            else if (qp == 2) { toL = true; toP = eq; }
+            else if (qp == 3) { toL = true; toP = false; }
+            else { toL = false; toP = true; }
+            toL = toL && needL;
+            toP = toP && needP;
+            if (!toL && !toP) continue;
+            const R val = shape_eval<R, true>(dr + qp * hq, f.shp[j], f.ctl);
+            if (toL) aL.add(val);
+            if (toP) aP.add(val);
+        }
+    }
+}
+
+// direct deposits at fine point i (1-based record index, position (i-1) H)
+template <typename R>
+__device__ inline void gather_fine_node(const FineCtx<R> &f, int i, bool left_tile, bool right_tile, Acc<R> &acc) {
+    const double xi = (double)(i - 1) * HF;
+    {   // centre lines, left part of the core: u in [(i-1)H, (floor((i-2)/4) + 2) H9)
+        const double uhi = (double)(((i - 2) >> 2) + 2) * H9;
+        int j0, j1;
+        cand(f, xi, uhi, j0, j1);
+        for (int j = j0; j < j1; j++) {
+            double u;
+            if (classify(f, j, u) != 2 || u < xi || !(u < uhi)) continue;
+            acc.add(shape_eval<R, true>(u - xi, f.shp[j], f.ctl));
+        }
+    }
+    {   // right part: u < (i-1)H and v < (floor((NT-i)/4) + 2) H9
+        const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
+        int j0, j1;
+        cand(f, 10.0 - vhi - 1e-9, xi, j0, j1);
+        for (int j = j0; j < j1; j++) {
+            double u;
+            if (classify(f, j, u) != 2 || !(u < xi) || !(10.0 - u < vhi)) continue;
+            acc.add(shape_eval<R, true>(u - xi, f.shp[j], f.ctl));
+        }
+    }
+    if (left_tile && i <= 5) {  // leftLBL :107-112 (d < H9)
+        int j0, j1;
+        cand(f, -H9, 0.0, j0, j1);
+        for (int j = j0; j < j1; j++) {
+            double u;
+            if (classify(f, j, u) != 1 || !(-u < H9)) continue;
+            const double X = i == 2 ? u - HF : i == 3 ? (u - HF) - HF : i == 4 ? (u + HF) - H9 : u - H9;
+            acc.add(shape_eval<R, true>(X, f.shp[j], f.ctl));
+        }
+    }
+    if (right_tile && i >= NT - 3) {  // rightLBL :592-595 (d < H9)
+        int j0, j1;
+        cand(f, 10.0 - 1e-9, 10.0 + H9 + 1e-9, j0, j1);
+        for (int j = j0; j < j1; j++) {
+            double u;
+            if (classify(f, j, u) != 3) continue;
+            const double dr = u - 10.0;
+            if (!(dr < H9)) continue;
+            const double X = i == NT ? dr : i == NT - 1 ? dr + HF : i == NT - 2 ? (dr + HF) + HF : (dr + H9) - HF;
+            acc.add(shape_eval<R, true>(X, f.shp[j], f.ctl));
+        }
+    }
+}
+
+// offset of grid k's position nodes in the per-tile node list, k = 1..9
+__device__ inline int lvl_node_off(int k) { return (1 << (k + 1)) - 4 + (k - 1); }
+// offset of grid k's P block in the tile's node sums; mid at +2^k, L at +2*2^k
+__device__ inline int lvl_acc_off(int k) { return 3 * ((1 << k) - 2); }
+constexpr int FINE_OFF = 3066;
+constexpr int WAVE_NODES = 64;  // position nodes of grids 1..4
+
+__device__ inline void store_level_node(double *g, int k, int t, int qloc, double vL, double vP) {
+    const int nk = 1 << k, span = 2 << k, accb = lvl_acc_off(k);
+    if (qloc & 1) {
+        g[accb + nk + ((qloc + 1) >> 1) - 1] = vL;
+    } else {
+        if (qloc > 0) g[accb + 2 * nk + (qloc >> 1) - 1] = vL;
+        if (qloc < span) g[accb + (qloc >> 1)] = vP;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
+    __shared__ double g[NACC];
+    const int nitems = a.nlev * a.nrec * TILES;
+    const int w = xcd_item(nitems);
+    if (w >= nitems) return;
+    const int r = w / TILES, t = w % TILES;
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const double sc = a.sscale[a.lev0 + lev];
+    FineCtx<R> f;
+    f.skey = a.skey + (int64_t)lev * a.nl;
+    f.sidx = a.sidx + (int64_t)lev * a.nl;
+    f.shp = reinterpret_cast<const LineShape<R> *>(a.shp) + (int64_t)lev * a.nl;
+    f.bst = a.bst + (int64_t)lev * (a.NB + 1);
+    f.org = a.org[s];
+    f.top = f.org + 10.0;
+    f.base = a.bbase;
+    f.cut = (double)a.cutoff;
+    f.NB = a.NB;
+    f.wlo = a.lo[s];
+    f.whi = a.hi[s];
+    f.ctl = make_ctl(a, a.lev0 + lev);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+    // grids 1..4: one wavefront per node, lines split over the lanes, fixed-order reduction
+    for (int nd = wv; nd < WAVE_NODES; nd += BLOCK / 64) {
+        int k = 1;
+        while (k < 4 && nd >= lvl_node_off(k + 1)) k++;
+        const int qloc = nd - lvl_node_off(k), span = 2 << k;
+        Acc<R> aL, aP;
+        aL.zero();
+        aP.zero();
+        gather_level_node<R>(f, k, t * span + qloc, (qloc & 1) || qloc > 0, !(qloc & 1) && qloc < span, lane, 64,
+                             aL, aP);
+        double vL = aL.value(), vP = aP.value();
+        for (int off = 32; off > 0; off >>= 1) {
+            vL += __shfl_xor(vL, off);
+            vP += __shfl_xor(vP, off);
+        }
+        if (lane == 0) store_level_node(g, k, t, qloc, vL, vP);
+    }
+    // grids 5..9 and the fine points: one thread per node
+    for (int nd = WAVE_NODES + tid; nd < NNODES; nd += BLOCK) {
+        Acc<R> aL, aP;
+        aL.zero();
+        aP.zero();
+        if (nd < NLVL_NODES) {
+            int k = 5;
+            while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
+            const int qloc = nd - lvl_node_off(k), span = 2 << k;
+            gather_level_node<R>(f, k, t * span + qloc, (qloc & 1) || qloc > 0, !(qloc & 1) && qloc < span, 0, 1,
+                                 aL, aP);
+            store_level_node(g, k, t, qloc, aL.value(), aP.value());
+        } else {
+            const int il = nd - NLVL_NODES;
+            gather_fine_node<R>(f, 2048 * t + 2 + il, t == 0, t == TILES - 1, aL);
+            g[FINE_OFF + il] = aL.value();
+        }
+    }
+    __syncthreads();
+
+    const double *cz = a.coarse + (int64_t)r * NCOARSE;
+    if (a.grids && r == 0) {
+        for (int e = tid; e < NACC; e += BLOCK) {
+            int dst;
+            if (e >= FINE_OFF) {
+                dst = 30690 + 2048 * t + 1 + (e - FINE_OFF);
+            } else {
+                int k = 1;
+                while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
+                const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
+                dst = 30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl;
+            }
+            a.grids[dst] = g[e] / sc;
+        }
+        if (tid < 3) a.grids[tid * 10 + t] = cz[1 + 10 * tid + t] / sc;
+        if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
+    }
+
+    // ---- cascade of this tile in FP64 (src/Grids.f90:80-139)
+    auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i) {
+        fP[i] += P;
+        double A = 0.375 * P + 0.75 * M - 0.125 * L;
+        if (A < P || A < M) A = 0.5 * (P + M);
+        fM[i] += A;
+        fL[i] += M;
+        fP[i + 1] += M;
+        A = 0.375 * L + 0.75 * M - 0.125 * P;
+        if (A < L || A < M) A = 0.5 * (L + M);
+        fM[i + 1] += A;
+        fL[i + 1] += L;
+    };
+    if (tid == 0) {
+        const int k1 = lvl_acc_off(1);
+        step(cz[1 + t], cz[11 + t], cz[21 + t], g + k1, g + k1 + 2, g + k1 + 4, 0);
+    }
+    __syncthreads();
+    for (int k = 1; k < 9; k++) {
+        const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
+        for (int c = tid; c < nk; c += BLOCK)
+            step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c);
+        __syncthreads();
+    }
+    {
+        const int cb = lvl_acc_off(9);
+        double *rk = g + FINE_OFF;
+        for (int c = tid; c < 512; c += BLOCK) {
+            const double P = g[cb + c], M = g[cb + 512 + c], L = g[cb + 1024 + c];
+            double A = P * 0.375 + M * 0.75 - L * 0.125;
+            if (A < P || A < M) A = 0.5 * (P + M);
+            rk[4 * c] += A;
+            rk[4 * c + 1] += M;
+            A = L * 0.375 + M * 0.75 - P * 0.125;
+            if (A < L || A < M) A = 0.5 * (L + M);
+            rk[4 * c + 2] += A;
+            rk[4 * c + 3] += L;
+        }
+    }
+    __syncthreads();
+    const double mul = (double)a.outmul[a.lev0 + lev] / sc;
+    float *out = a.out + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
+    for (int e = tid; e < 2048; e += BLOCK) out[2048 * t + 1 + e] = (float)(g[FINE_OFF + e] * mul);
+    if (t == 0 && tid == 0) out[0] = (float)(cz[0] * mul);
+}

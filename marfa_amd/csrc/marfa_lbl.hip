@@ -60,11 +60,8 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int TILES = 10;      // 1 cm-1 tiles per 10 cm-1 record
-constexpr int CH = BLOCK;      // candidate lines staged per chunk
 constexpr int NCOARSE = 32;    // RK1, P[10], M[10], L[10], pad
 constexpr int NACC = 5114;     // per tile: 3*(2+4+...+512) grid nodes + 2048 fine points
-constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes
-constexpr int NNODES = NLVL_NODES + 2048;
 
 enum : uint8_t { CLS_NONE = 0, CLS_LEFT = 1, CLS_CENTER = 2, CLS_RIGHT = 3 };
 
@@ -86,6 +83,10 @@ struct KArgs {
     const double *nus, *gD, *gL, *S;          // [nlev][nl]
     const uint64_t *skey;                     // [nlev][nl] sorted keys of shifted centres
     const int32_t *sidx;                      // [nlev][nl] line index of sorted entry
+    const void *shp;                          // [nlev][nl] LineShape<R> of sorted entries
+    const int32_t *bst;                       // [nlev][NB+1] first sorted entry of each bucket
+    double bbase;                             // shifted centre of bucket 0's left edge
+    int NB;                                   // buckets (1/512 cm-1) per level
     const int64_t *lo, *hi;                   // [nrec] windows (global line indices)
     const double *org;                        // [nrec] subinterval origins
     const double *sscale;                     // [lev] sample scale (power of 2)
@@ -296,334 +297,9 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
 }
 
 // ------------------------------------------------------------------ 4. fine + cascade
-__device__ inline int lb_ge(const double *x, int n, double v) {  // first j with x[j] >= v
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int m = (lo + hi) >> 1;
-        if (x[m] < v) lo = m + 1; else hi = m;
-    }
-    return lo;
-}
-__device__ inline int lb_gt(const double *x, int n, double v) {  // first j with x[j] > v
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int m = (lo + hi) >> 1;
-        if (!(x[m] > v)) lo = m + 1; else hi = m;
-    }
-    return lo;
-}
-__device__ inline int lb_vlt(const double *x, int n, double v) {  // first j with x[j] < v (x non-increasing)
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int m = (lo + hi) >> 1;
-        if (!(x[m] < v)) lo = m + 1; else hi = m;
-    }
-    return lo;
-}
-
-// offset of level k's position nodes in the per-tile node list, k = 1..9
-__device__ inline int lvl_node_off(int k) { return (1 << (k + 1)) - 4 + (k - 1); }
-// offset of level k's P block in the tile accumulators; M at +2^k, L at +2*2^k
-__device__ inline int lvl_acc_off(int k) { return 3 * ((1 << k) - 2); }
-constexpr int FINE_OFF = 3066;
-
-template <typename R>
-struct FineShared {
-    Acc<R> acc[NACC];
-    double u[CH], v[CH];
-    LineShape<R> ln[CH];
-    uint8_t cls[CH];
-    int64_t crange[2];
-};
-
-template <typename R>
-__device__ inline void add_acc(FineShared<R> &sh, int slot, R val) { sh.acc[slot].add(val); }
-
-template <typename R>
-__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
-    __shared__ FineShared<R> sh;
-    const int nitems = a.nlev * a.nrec * TILES;
-    const int w = xcd_item(nitems);
-    if (w >= nitems) return;
-    const int r = w / TILES, t = w % TILES;
-    const int lev = r / a.nrec, s = r % a.nrec;
-    const double org = a.org[s];
-    const int64_t wlo = a.lo[s], whi = a.hi[s];
-    const double cut = (double)a.cutoff;
-    const double top = org + 10.0;
-    const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
-    const double sc = a.sscale[a.lev0 + lev];
-    const int tid = threadIdx.x;
-
-    for (int e = tid; e < NACC; e += BLOCK) sh.acc[e].zero();
-    // candidate lines: shifted centre within [org + t - 2, org + t + 3] (every deposit of
-    // grids 1..9 and of the fine points into this tile comes from there)
-    const uint64_t *skey = a.skey + (int64_t)lev * a.nl;
-    if (tid < 2) {
-        const uint64_t x = key_of(tid == 0 ? org + (t - 2) - 1e-9 : org + (t + 3) + 1e-9);
-        int64_t lo = 0, hi = a.nl;
-        while (lo < hi) {
-            int64_t m = (lo + hi) >> 1;
-            if (skey[m] < x) lo = m + 1; else hi = m;
-        }
-        sh.crange[tid] = lo;
-    }
-    __syncthreads();
-    const int64_t c0 = sh.crange[0], c1 = sh.crange[1];
-    const int64_t base = (int64_t)lev * a.nl - a.lmin;
-
-    for (int64_t cb = c0; cb < c1; cb += CH) {
-        const int cn = (int)min((int64_t)CH, c1 - cb);
-        if (tid < cn) {
-            const int64_t j = cb + tid;
-            const double nus = double_of(skey[j]);
-            const int64_t li = a.sidx[(int64_t)lev * a.nl + j];
-            const double u = nus - org;
-            uint8_t cls = CLS_NONE;
-            if (li >= wlo && li < whi) {
-                if (nus < org) {
-                    const double d = -u;
-                    if (!(d > cut) && d < 1.0) cls = CLS_LEFT;
-                } else if (nus > top) {
-                    const double dr = u - 10.0;
-                    if (!(dr >= cut) && dr < 1.0) cls = CLS_RIGHT;
-                } else if (u < 10.0) {
-                    cls = CLS_CENTER;
-                }
-            }
-            sh.u[tid] = u;
-            sh.v[tid] = 10.0 - u;
-            sh.cls[tid] = cls;
-            if (cls != CLS_NONE) sh.ln[tid] = make_line<R>(a.gD[base + li], a.gL[base + li], a.S[base + li] * sc);
-        }
-        __syncthreads();
-        const double umin = sh.u[0], umax = sh.u[cn - 1];
-
-        for (int nd = tid; nd < NNODES; nd += BLOCK) {
-            if (nd < NLVL_NODES) {
-                // ---- grid level k, position q (spacing H_k/2)
-                int k = 1;
-                while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
-                const int qloc = nd - lvl_node_off(k);
-                const int span = 2 << k;              // positions per tile
-                const int q = t * span + qloc;
-                const int NTk = 10 << k;
-                const double Hk = hk(k), hq = 0.5 * Hk;
-                const double xq = q * hq;
-                const int cellbase = t << k;          // cells before the tile
-                const int accb = lvl_acc_off(k), nk = 1 << k;
-                // center lines, left walk: floor(u 2^k) in [nlo, nhi]
-                {
-                    const int nlo = max(2, ((q + 1) >> 1) + 1), nhi = 2 * (q >> 2) + 3;
-                    const double ulo = nlo * Hk, uhi = (nhi + 1) * Hk;
-                    if (umax >= ulo && umin < uhi) {
-                        const int j0 = lb_ge(sh.u, cn, ulo), j1 = lb_ge(sh.u, cn, uhi);
-                        for (int j = j0; j < j1; j++) {
-                            if (sh.cls[j] != CLS_CENTER) continue;
-                            const double u = sh.u[j];
-                            const int n = (int)floor(__builtin_ldexp(u, k));
-                            const int lo_q = 4 * max(0, (n >> 1) - 1), hi_q = 2 * (n - 1);
-                            if (q < lo_q || q > hi_q) continue;
-                            const R val = shape_eval<R>(u - xq, sh.ln[j], ctl);
-                            if (q & 1) {
-                                sh.acc[accb + nk + ((q + 1) >> 1) - 1 - cellbase].add(val);
-                            } else {
-                                if (q > lo_q && qloc > 0) sh.acc[accb + 2 * nk + (q >> 1) - 1 - cellbase].add(val);
-                                if (q < hi_q && qloc < span) sh.acc[accb + (q >> 1) - cellbase].add(val);
-                            }
-                        }
-                    }
-                }
-                // center lines, right walk: floor(v 2^k) in [nlo, nhi] for q' = 2 NT_k - q
-                {
-                    const int qp = 2 * NTk - q;
-                    const int nlo = max(2, ((qp + 1) >> 1) + 1), nhi = 2 * (qp >> 2) + 3;
-                    const double vlo = nlo * Hk, vhi = (nhi + 1) * Hk;
-                    // v = 10 - u is non-increasing along the chunk
-                    if (sh.v[0] >= vlo && sh.v[cn - 1] < vhi) {
-                        const int j0 = lb_vlt(sh.v, cn, vhi), j1 = lb_vlt(sh.v, cn, vlo);
-                        for (int j = j0; j < j1; j++) {
-                            if (sh.cls[j] != CLS_CENTER) continue;
-                            const double v = sh.v[j];
-                            const int n = (int)floor(__builtin_ldexp(v, k));
-                            const int lo_q = 2 * NTk - 2 * (n - 1), hi_q = 2 * NTk - 4 * max(0, (n >> 1) - 1);
-                            if (q < lo_q || q > hi_q) continue;
-                            const R val = shape_eval<R>(v - qp * hq, sh.ln[j], ctl);
-                            if (q & 1) {
-                                sh.acc[accb + nk + ((q + 1) >> 1) - 1 - cellbase].add(val);
-                            } else {
-                                if (q > lo_q && qloc > 0) sh.acc[accb + 2 * nk + (q >> 1) - 1 - cellbase].add(val);
-                                if (q < hi_q && qloc < span) sh.acc[accb + (q >> 1) - cellbase].add(val);
-                            }
-                        }
-                    }
-                }
-                // wing lines within H_{k-1} of the left edge: cells 1 and 2 (leftLBL :34-164)
-                if (t == 0 && q <= 4 && umin < 0.0) {
-                    const double hkm1 = hk(k - 1);
-                    const int j0 = lb_gt(sh.u, cn, -hkm1), j1 = lb_ge(sh.u, cn, 0.0);
-                    for (int j = j0; j < j1; j++) {
-                        if (sh.cls[j] != CLS_LEFT) continue;
-                        const double u = sh.u[j];
-                        const bool eq = (-u >= Hk);  // m == k (else m > k)
-                        if (!eq && q < 2) continue;
-                        const R val = shape_eval<R>(u - xq, sh.ln[j], ctl);
-                        // cells 1 and 2 are local cells 0 and 1 of tile 0
-                        if (q == 0) sh.acc[accb + 0].add(val);
-                        else if (q == 1) sh.acc[accb + nk + 0].add(val);
-                        else if (q == 2) {
-                            if (eq) sh.acc[accb + 2 * nk + 0].add(val);
-                            sh.acc[accb + 1].add(val);
-                        } else if (q == 3) sh.acc[accb + nk + 1].add(val);
-                        else sh.acc[accb + 2 * nk + 1].add(val);
-                    }
-                }
-                // wing lines within H_{k-1} of the right edge: cells NT_k and NT_k - 1
-                if (t == TILES - 1 && 2 * NTk - q <= 4 && umax > 10.0 - 1e-9) {
-                    const int qp = 2 * NTk - q;
-                    const double hkm1 = hk(k - 1);
-                    const int j0 = lb_gt(sh.u, cn, 10.0 - 1e-9);
-                    for (int j = j0; j < cn; j++) {
-                        if (sh.cls[j] != CLS_RIGHT) continue;
-                        const double dr = sh.u[j] - 10.0;
-                        if (!(dr < hkm1)) break;
-                        const bool eq = (dr >= Hk);
-                        if (!eq && qp < 2) continue;
-                        const R val = shape_eval<R>(dr + qp * hq, sh.ln[j], ctl);
-                        const int cN = nk - 1, cN1 = nk - 2;  // local cells of NT_k, NT_k - 1
-                        if (qp == 0) sh.acc[accb + 2 * nk + cN].add(val);
-                        else if (qp == 1) sh.acc[accb + nk + cN].add(val);
-                        else if (qp == 2) {
-                            if (eq) sh.acc[accb + cN].add(val);
-                            sh.acc[accb + 2 * nk + cN1].add(val);
-                        } else if (qp == 3) sh.acc[accb + nk + cN1].add(val);
-                        else sh.acc[accb + cN1].add(val);
-                    }
-                }
-            } else {
-                // ---- fine point i (1-based record index), position (i-1) H
-                const int il = nd - NLVL_NODES;
-                const int i = 2048 * t + 2 + il;
-                const double xi = (double)(i - 1) * HF;
-                Acc<R> &acc = sh.acc[FINE_OFF + il];
-                // centre lines, left part: i in [4 e9 + 2, floor(u/H) + 1] (centerLBL :332-339)
-                {
-                    const double uhi = (double)(((i - 2) >> 2) + 2) * H9;
-                    if (umax >= xi && umin < uhi) {
-                        const int j0 = lb_ge(sh.u, cn, xi), j1 = lb_ge(sh.u, cn, uhi);
-                        for (int j = j0; j < j1; j++)
-                            if (sh.cls[j] == CLS_CENTER) acc.add(shape_eval<R>(sh.u[j] - xi, sh.ln[j], ctl));
-                    }
-                }
-                // right part: i in [floor(u/H) + 2, NT - 4 e9'] (centerLBL :485-490)
-                {
-                    const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
-                    if (umin < xi && sh.v[cn - 1] < vhi) {
-                        const int j0 = lb_vlt(sh.v, cn, vhi), j1 = lb_ge(sh.u, cn, xi);
-                        for (int j = j0; j < j1; j++)
-                            if (sh.cls[j] == CLS_CENTER) acc.add(shape_eval<R>(sh.u[j] - xi, sh.ln[j], ctl));
-                    }
-                }
-                // wing lines closer than H9 to an edge: four fine points (leftLBL :107-112,
-                // rightLBL :592-595)
-                if (t == 0 && i <= 5 && umin < 0.0) {
-                    const int j0 = lb_gt(sh.u, cn, -H9), j1 = lb_ge(sh.u, cn, 0.0);
-                    for (int j = j0; j < j1; j++) {
-                        if (sh.cls[j] != CLS_LEFT) continue;
-                        const double uu = sh.u[j];
-                        const double X = i == 2 ? uu - HF : i == 3 ? (uu - HF) - HF
-                                       : i == 4 ? (uu + HF) - H9 : uu - H9;
-                        acc.add(shape_eval<R>(X, sh.ln[j], ctl));
-                    }
-                }
-                if (t == TILES - 1 && i >= NT - 3 && umax > 10.0 - 1e-9) {
-                    const int j0 = lb_gt(sh.u, cn, 10.0 - 1e-9);
-                    for (int j = j0; j < cn; j++) {
-                        if (sh.cls[j] != CLS_RIGHT) continue;
-                        const double dr = sh.u[j] - 10.0;
-                        if (!(dr < H9)) break;
-                        const double X = i == NT ? dr : i == NT - 1 ? dr + HF
-                                       : i == NT - 2 ? (dr + HF) + HF : (dr + H9) - HF;
-                        acc.add(shape_eval<R>(X, sh.ln[j], ctl));
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-
-    // ---- cascade of this tile in FP64 (src/Grids.f90:80-139)
-    double *g = reinterpret_cast<double *>(sh.acc);
-    static_assert(sizeof(Acc<R>) == sizeof(double), "accumulator slot must be 8 bytes");
-    for (int e = tid; e < NACC; e += BLOCK) {
-        const double v = sh.acc[e].value();
-        __builtin_memcpy(&g[e], &v, 8);
-    }
-    __syncthreads();
-    const double *cz = a.coarse + (int64_t)r * NCOARSE;
-    if (a.grids && r == 0) {
-        for (int e = tid; e < NACC; e += BLOCK) {
-            int dst;
-            if (e >= FINE_OFF) {
-                dst = 30690 + 2048 * t + 1 + (e - FINE_OFF);
-            } else {
-                int k = 1;
-                while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
-                const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
-                dst = 30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl;
-            }
-            a.grids[dst] = g[e] / sc;
-        }
-        if (tid < 3) {
-            const int dst = tid * 10 + t;  // level-0 P|mid|L of cell t+1
-            a.grids[dst] = cz[1 + 10 * tid + t] / sc;
-        }
-        if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
-    }
-    auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i) {
-        fP[i] += P;
-        double A = 0.375 * P + 0.75 * M - 0.125 * L;
-        if (A < P || A < M) A = 0.5 * (P + M);
-        fM[i] += A;
-        fL[i] += M;
-        fP[i + 1] += M;
-        A = 0.375 * L + 0.75 * M - 0.125 * P;
-        if (A < L || A < M) A = 0.5 * (L + M);
-        fM[i + 1] += A;
-        fL[i + 1] += L;
-    };
-    if (tid == 0) {
-        const int k1 = lvl_acc_off(1);
-        step(cz[1 + t], cz[11 + t], cz[21 + t], g + k1, g + k1 + 2, g + k1 + 4, 0);
-    }
-    __syncthreads();
-    for (int k = 1; k < 9; k++) {
-        const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
-        for (int c = tid; c < nk; c += BLOCK)
-            step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c);
-        __syncthreads();
-    }
-    {
-        const int cb = lvl_acc_off(9);
-        double *rk = g + FINE_OFF;
-        for (int c = tid; c < 512; c += BLOCK) {
-            const double P = g[cb + c], M = g[cb + 512 + c], L = g[cb + 1024 + c];
-            double A = P * 0.375 + M * 0.75 - L * 0.125;
-            if (A < P || A < M) A = 0.5 * (P + M);
-            rk[4 * c] += A;
-            rk[4 * c + 1] += M;
-            A = L * 0.375 + M * 0.75 - P * 0.125;
-            if (A < L || A < M) A = 0.5 * (L + M);
-            rk[4 * c + 2] += A;
-            rk[4 * c + 3] += L;
-        }
-    }
-    __syncthreads();
-    const double mul = (double)a.outmul[a.lev0 + lev] / sc;
-    float *out = a.out + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
-    for (int e = tid; e < 2048; e += BLOCK) out[2048 * t + 1 + e] = (float)(g[FINE_OFF + e] * mul);
-    if (t == 0 && tid == 0) out[0] = (float)(cz[0] * mul);
-}
+constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes per tile
+constexpr int NNODES = NLVL_NODES + 2048;
+#include "lbl_fine.h"
 
 // ------------------------------------------------------------------ sample counts
 __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
@@ -711,7 +387,7 @@ struct marfa_ctx {
         void *p = nullptr;
         size_t bytes = 0;
     };
-    Buf params[4], keys[2], idx[2], sorttmp, coarse, lvl, win, out, counts;
+    Buf params[4], keys[2], idx[2], sorttmp, coarse, lvl, win, out, counts, shapes, buckets;
     hipEvent_t ev[5] = {};
     float timings[5] = {0, 0, 0, 0, 0};
     double *dbg_grids = nullptr;
@@ -796,8 +472,14 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     const int64_t nl = std::max<int64_t>(lmax - lmin, 1);
     if (lmin < 0 || lmax > c->n) return set_err(c, MARFA_E_WINDOW, "window outside the line list");
 
+    // bucket index of shifted centres: 1/512 cm-1 buckets over every record's reach
+    const double bbase = W.org.front() - (double)job.cutoff - 4.0;
+    const double btop = W.org.back() + 10.0 + (double)job.cutoff + 4.0;
+    const int NB = (int)std::ceil((btop - bbase) * 512.0) + 1;
+    const size_t shp_bytes = job.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
+
     // level batches so that per-(line, level) arrays stay within a memory budget
-    const size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4);
+    const size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(NB + 1) * 4;
     const size_t budget = (size_t)24 << 30;
     int lb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nlev, budget / std::max<size_t>(per_level, 1)));
 
@@ -825,6 +507,8 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         if (ensure(c, c->idx[q], arr * 4)) return MARFA_E_HIP;
     }
     if (ensure(c, c->coarse, (size_t)lb * nrec * NCOARSE * 8)) return MARFA_E_HIP;
+    if (ensure(c, c->shapes, arr * shp_bytes)) return MARFA_E_HIP;
+    if (ensure(c, c->buckets, (size_t)lb * (NB + 1) * 4)) return MARFA_E_HIP;
     size_t tmpb = 0;
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, (uint64_t *)nullptr, (uint64_t *)nullptr,
                                        (int32_t *)nullptr, (int32_t *)nullptr, (int)nl, 0, 64, st);
@@ -889,6 +573,22 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
                 (int32_t *)c->idx[1].p + (size_t)q * nl, (int)nl, 0, 64, st);
             if (e != hipSuccess) return set_err(c, MARFA_E_HIP, std::string("radix sort: ") + hipGetErrorString(e));
         }
+        {
+            dim3 gsh((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
+            if (job.precision == MARFA_FP32K)
+                hipLaunchKernelGGL(shape_kernel<float>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->idx[1].p, nl,
+                                   lmin, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
+                                   (const double *)dsc, l0, (LineShape<float> *)c->shapes.p);
+            else
+                hipLaunchKernelGGL(shape_kernel<double>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->idx[1].p, nl,
+                                   lmin, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
+                                   (const double *)dsc, l0, (LineShape<double> *)c->shapes.p);
+            HIPCHK(hipGetLastError());
+            dim3 gb((unsigned)((nl + 1 + BLOCK - 1) / BLOCK), (unsigned)nb);
+            hipLaunchKernelGGL(bucket_kernel, gb, dim3(BLOCK), 0, st, (const uint64_t *)c->keys[1].p, nl, bbase, NB,
+                               (int32_t *)c->buckets.p);
+            HIPCHK(hipGetLastError());
+        }
         if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
         KArgs ka;
         ka.nlev = nb;
@@ -915,6 +615,10 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.coarse = (double *)c->coarse.p;
         ka.out = out;
         ka.grids = c->dbg_grids;
+        ka.shp = c->shapes.p;
+        ka.bst = (const int32_t *)c->buckets.p;
+        ka.bbase = bbase;
+        ka.NB = NB;
         const int nitems = nb * nrec;
         const unsigned gcoarse = (unsigned)(8 * ((nitems + 7) / 8));
         const unsigned gfine = (unsigned)(8 * ((nitems * TILES + 7) / 8));
@@ -980,7 +684,7 @@ int marfa_ctx_destroy(marfa_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     marfa_ctx::Buf *bufs[] = {&c->params[0], &c->params[1], &c->params[2], &c->params[3], &c->keys[0], &c->keys[1],
-                              &c->idx[0], &c->idx[1], &c->sorttmp, &c->coarse, &c->lvl, &c->win, &c->out, &c->counts};
+                              &c->idx[0], &c->idx[1], &c->sorttmp, &c->coarse, &c->lvl, &c->win, &c->out, &c->counts, &c->shapes, &c->buckets};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
