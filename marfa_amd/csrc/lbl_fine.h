@@ -24,17 +24,32 @@ __device__ inline int bucket_of(double nus, double base, int NB) {
     return (int)x;
 }
 
-// bst[b] = first sorted line whose bucket is >= b, b = 0..NB (each entry written once)
+// bst[b] = first sorted line whose bucket is >= b, b = 0..NB (each entry written once).
+// Line j fills the buckets (bucket(j-1), bucket(j)]; gaps longer than a few buckets (empty
+// stretches of spectrum) are filled by the whole wavefront together.
 __global__ __launch_bounds__(BLOCK) void bucket_kernel(const uint64_t *skey, int64_t nl, double base, int NB,
                                                        int32_t *bst) {
     const int lev = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j > nl) return;
+    const int lane = threadIdx.x & 63;
     const uint64_t *k = skey + (int64_t)lev * nl;
     int32_t *o = bst + (int64_t)lev * (NB + 1);
-    const int bprev = j == 0 ? -1 : bucket_of(double_of(k[j - 1]), base, NB);
-    const int bcur = j == nl ? NB : bucket_of(double_of(k[j]), base, NB);
-    for (int b = bprev + 1; b <= bcur; b++) o[b] = (int32_t)j;
+    int b0 = 0, b1 = -1;  // this lane's gap [b0, b1]
+    if (j <= nl) {
+        b0 = (j == 0 ? -1 : bucket_of(double_of(k[j - 1]), base, NB)) + 1;
+        b1 = j == nl ? NB : bucket_of(double_of(k[j]), base, NB);
+    }
+    constexpr int SHORT = 8;
+    if (b1 - b0 < SHORT)
+        for (int b = b0; b <= b1; b++) o[b] = (int32_t)j;
+    uint64_t pending = __ballot(b1 - b0 >= SHORT);
+    while (pending) {
+        const int l = __builtin_ctzll(pending);
+        pending &= pending - 1;
+        const int s = __shfl(b0, l), e = __shfl(b1, l);
+        const int32_t v = (int32_t)__shfl((int)j, l);
+        for (int b = s + lane; b <= e; b += 64) o[b] = v;
+    }
 }
 
 // profile parameters of every line, in sorted order, for the fine gathers
@@ -92,11 +107,47 @@ __device__ inline int classify(const FineCtx<R> &f, int j, double &u) {
     return u < 10.0 ? 2 : 0;
 }
 
+// Candidate lines read straight from the level's sorted arrays in global memory.
+template <typename R>
+struct GlobalSrc {
+    const FineCtx<R> &f;
+    const ShapeCtl &ctl;
+    __device__ GlobalSrc(const FineCtx<R> &f_) : f(f_), ctl(f_.ctl) {}
+    __device__ inline void range(double u0, double u1, int &j0, int &j1) const { cand(f, u0, u1, j0, j1); }
+    __device__ inline int cls(int j, double &u) const { return classify(f, j, u); }
+    __device__ inline const LineShape<R> &shape(int j) const { return f.shp[j]; }
+};
+
+// Candidate lines of the tile staged in LDS: u and class per line, and the slice of the
+// bucket index covering the staged window (indices relative to the first staged line).
+template <typename R>
+struct LdsSrc {
+    const double *su;
+    const uint8_t *scls;
+    const int *sb;
+    int B0, nsb;
+    double org, base;
+    const LineShape<R> *shp;
+    ShapeCtl ctl;
+    __device__ inline void range(double u0, double u1, int &j0, int &j1) const {
+        const double x0 = (org + u0 - base) * BUCKETS_PER_CM, x1 = (org + u1 - base) * BUCKETS_PER_CM;
+        const int b0 = (int)fmin(fmax(floor(x0) - 1.0 - B0, 0.0), (double)nsb);
+        const int b1 = (int)fmin(fmax(floor(x1) + 2.0 - B0, 0.0), (double)nsb);
+        j0 = sb[b0];
+        j1 = b1 > b0 ? sb[b1] : j0;
+    }
+    __device__ inline int cls(int j, double &u) const {
+        u = su[j];
+        return scls[j];
+    }
+    __device__ inline const LineShape<R> &shape(int j) const { return shp[j]; }
+};
+
 // Sums at one position node of grid k: global position q in steps of H_k/2.  Even q is the
 // right edge (L) of cell q/2 -> aL and the left edge (P) of cell q/2+1 -> aP; odd q is the
 // midpoint of cell (q+1)/2 -> aL.  needL / needP: the roles this tile owns.
-template <typename R>
-__device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool needL, bool needP,
+template <typename R, typename Src>
+__device__ inline void gather_level_node(const Src &src, int k, int q, bool needL, bool needP,
                                          int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
     const int NTk = 10 << k;
     const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
@@ -106,16 +157,16 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
         const int nlo = max(2, ((q + 1) >> 1) + 1), nhi = 2 * (q >> 2) + 3;
         const double ulo = nlo * Hk, uhi = (nhi + 1) * Hk;
         int j0, j1;
-        cand(f, ulo, uhi, j0, j1);
+        src.range(ulo, uhi, j0, j1);
         for (int j = j0 + jfirst; j < j1; j += jstep) {
             double u;
-            if (classify(f, j, u) != 2 || u < ulo || !(u < uhi)) continue;
+            if (src.cls(j, u) != 2 || u < ulo || !(u < uhi)) continue;
             const int n = (int)floor(__builtin_ldexp(u, k));
             const int lo_q = 4 * max(0, (n >> 1) - 1), hi_q = 2 * (n - 1);
             if (q < lo_q || q > hi_q) continue;
             const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
             if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(u - xq, f.shp[j], f.ctl);
+            const R val = shape_eval<R, true>(u - xq, src.shape(j), src.ctl);
             if (toL) aL.add(val);
             if (toP) aP.add(val);
         }
@@ -126,10 +177,10 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
         const int nlo = max(2, ((qp + 1) >> 1) + 1), nhi = 2 * (qp >> 2) + 3;
         const double vlo = nlo * Hk, vhi = (nhi + 1) * Hk;
         int j0, j1;
-        cand(f, 10.0 - vhi - 1e-9, 10.0 - vlo + 1e-9, j0, j1);
+        src.range(10.0 - vhi - 1e-9, 10.0 - vlo + 1e-9, j0, j1);
         for (int j = j0 + jfirst; j < j1; j += jstep) {
             double u;
-            if (classify(f, j, u) != 2) continue;
+            if (src.cls(j, u) != 2) continue;
             const double v = 10.0 - u;
             if (v < vlo || !(v < vhi)) continue;
             const int n = (int)floor(__builtin_ldexp(v, k));
@@ -137,7 +188,7 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
             if (q < lo_q || q > hi_q) continue;
             const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
             if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(v - qp * hq, f.shp[j], f.ctl);
+            const R val = shape_eval<R, true>(v - qp * hq, src.shape(j), src.ctl);
             if (toL) aL.add(val);
             if (toP) aP.add(val);
         }
@@ -146,10 +197,10 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
     if (q <= 4) {
         const double hkm1 = hk(k - 1);
         int j0, j1;
-        cand(f, -hkm1, 0.0, j0, j1);
+        src.range(-hkm1, 0.0, j0, j1);
         for (int j = j0 + jfirst; j < j1; j += jstep) {
             double u;
-            if (classify(f, j, u) != 1) continue;
+            if (src.cls(j, u) != 1) continue;
             const double d = -u;
             if (!(d < hkm1)) continue;
             const bool eq = d >= Hk;
@@ -161,7 +212,7 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
             toL = toL && needL;
             toP = toP && needP;
             if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(u - xq, f.shp[j], f.ctl);
+            const R val = shape_eval<R, true>(u - xq, src.shape(j), src.ctl);
             if (toL) aL.add(val);
             if (toP) aP.add(val);
         }
@@ -171,10 +222,10 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
         const int qp = 2 * NTk - q;
         const double hkm1 = hk(k - 1);
         int j0, j1;
-        cand(f, 10.0 - 1e-9, 10.0 + hkm1 + 1e-9, j0, j1);
+        src.range(10.0 - 1e-9, 10.0 + hkm1 + 1e-9, j0, j1);
         for (int j = j0 + jfirst; j < j1; j += jstep) {
             double u;
-            if (classify(f, j, u) != 3) continue;
+            if (src.cls(j, u) != 3) continue;
             const double dr = u - 10.0;
             if (!(dr < hkm1)) continue;
             const bool eq = dr >= Hk;
@@ -187,7 +238,7 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
             toL = toL && needL;
             toP = toP && needP;
             if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(dr + qp * hq, f.shp[j], f.ctl);
+            const R val = shape_eval<R, true>(dr + qp * hq, src.shape(j), src.ctl);
             if (toL) aL.add(val);
             if (toP) aP.add(val);
         }
@@ -195,49 +246,49 @@ __device__ inline void gather_level_node(const FineCtx<R> &f, int k, int q, bool
 }
 
 // direct deposits at fine point i (1-based record index, position (i-1) H)
-template <typename R>
-__device__ inline void gather_fine_node(const FineCtx<R> &f, int i, bool left_tile, bool right_tile, Acc<R> &acc) {
+template <typename R, typename Src>
+__device__ inline void gather_fine_node(const Src &src, int i, bool left_tile, bool right_tile, Acc<R> &acc) {
     const double xi = (double)(i - 1) * HF;
     {   // centre lines, left part of the core: u in [(i-1)H, (floor((i-2)/4) + 2) H9)
         const double uhi = (double)(((i - 2) >> 2) + 2) * H9;
         int j0, j1;
-        cand(f, xi, uhi, j0, j1);
+        src.range(xi, uhi, j0, j1);
         for (int j = j0; j < j1; j++) {
             double u;
-            if (classify(f, j, u) != 2 || u < xi || !(u < uhi)) continue;
-            acc.add(shape_eval<R, true>(u - xi, f.shp[j], f.ctl));
+            if (src.cls(j, u) != 2 || u < xi || !(u < uhi)) continue;
+            acc.add(shape_eval<R, true>(u - xi, src.shape(j), src.ctl));
         }
     }
     {   // right part: u < (i-1)H and v < (floor((NT-i)/4) + 2) H9
         const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
         int j0, j1;
-        cand(f, 10.0 - vhi - 1e-9, xi, j0, j1);
+        src.range(10.0 - vhi - 1e-9, xi, j0, j1);
         for (int j = j0; j < j1; j++) {
             double u;
-            if (classify(f, j, u) != 2 || !(u < xi) || !(10.0 - u < vhi)) continue;
-            acc.add(shape_eval<R, true>(u - xi, f.shp[j], f.ctl));
+            if (src.cls(j, u) != 2 || !(u < xi) || !(10.0 - u < vhi)) continue;
+            acc.add(shape_eval<R, true>(u - xi, src.shape(j), src.ctl));
         }
     }
     if (left_tile && i <= 5) {  // leftLBL :107-112 (d < H9)
         int j0, j1;
-        cand(f, -H9, 0.0, j0, j1);
+        src.range(-H9, 0.0, j0, j1);
         for (int j = j0; j < j1; j++) {
             double u;
-            if (classify(f, j, u) != 1 || !(-u < H9)) continue;
+            if (src.cls(j, u) != 1 || !(-u < H9)) continue;
             const double X = i == 2 ? u - HF : i == 3 ? (u - HF) - HF : i == 4 ? (u + HF) - H9 : u - H9;
-            acc.add(shape_eval<R, true>(X, f.shp[j], f.ctl));
+            acc.add(shape_eval<R, true>(X, src.shape(j), src.ctl));
         }
     }
     if (right_tile && i >= NT - 3) {  // rightLBL :592-595 (d < H9)
         int j0, j1;
-        cand(f, 10.0 - 1e-9, 10.0 + H9 + 1e-9, j0, j1);
+        src.range(10.0 - 1e-9, 10.0 + H9 + 1e-9, j0, j1);
         for (int j = j0; j < j1; j++) {
             double u;
-            if (classify(f, j, u) != 3) continue;
+            if (src.cls(j, u) != 3) continue;
             const double dr = u - 10.0;
             if (!(dr < H9)) continue;
             const double X = i == NT ? dr : i == NT - 1 ? dr + HF : i == NT - 2 ? (dr + HF) + HF : (dr + H9) - HF;
-            acc.add(shape_eval<R, true>(X, f.shp[j], f.ctl));
+            acc.add(shape_eval<R, true>(X, src.shape(j), src.ctl));
         }
     }
 }
@@ -283,16 +334,21 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     f.ctl = make_ctl(a, a.lev0 + lev);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
+    // Candidates are read from the level's sorted arrays through the bucket index (L2-resident
+    // for neighbouring tiles).  Staging them in LDS (LdsSrc) measured slower on config 2: the
+    // extra 23 KB of LDS halves the resident waves of this latency-bound gather.
+    const GlobalSrc<R> gsrc(f);
+
     // grids 1..4: one wavefront per node, lines split over the lanes, fixed-order reduction
     for (int nd = wv; nd < WAVE_NODES; nd += BLOCK / 64) {
         int k = 1;
         while (k < 4 && nd >= lvl_node_off(k + 1)) k++;
         const int qloc = nd - lvl_node_off(k), span = 2 << k;
+        const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
         Acc<R> aL, aP;
         aL.zero();
         aP.zero();
-        gather_level_node<R>(f, k, t * span + qloc, (qloc & 1) || qloc > 0, !(qloc & 1) && qloc < span, lane, 64,
-                             aL, aP);
+        gather_level_node<R>(gsrc, k, t * span + qloc, nL, nP, lane, 64, aL, aP);
         double vL = aL.value(), vP = aP.value();
         for (int off = 32; off > 0; off >>= 1) {
             vL += __shfl_xor(vL, off);
@@ -309,12 +365,12 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
             int k = 5;
             while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
             const int qloc = nd - lvl_node_off(k), span = 2 << k;
-            gather_level_node<R>(f, k, t * span + qloc, (qloc & 1) || qloc > 0, !(qloc & 1) && qloc < span, 0, 1,
-                                 aL, aP);
+            const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
+            gather_level_node<R>(gsrc, k, t * span + qloc, nL, nP, 0, 1, aL, aP);
             store_level_node(g, k, t, qloc, aL.value(), aP.value());
         } else {
             const int il = nd - NLVL_NODES;
-            gather_fine_node<R>(f, 2048 * t + 2 + il, t == 0, t == TILES - 1, aL);
+            gather_fine_node<R>(gsrc, 2048 * t + 2 + il, t == 0, t == TILES - 1, aL);
             g[FINE_OFF + il] = aL.value();
         }
     }

@@ -53,27 +53,7 @@ static inline float f_of_bits(uint32_t b) {
 #define TREF_F 296.f             /* Constants.f90:24 */
 
 static const uint32_t WISO_BITS[124] = {
-    0x419015A3U, 0x41A01E55U, 0x41981E45U, 0x41982249U, 0x41A82AFAU, 0x41A02AEBU,
-    0x41A00000U, 0x41B00000U, 0x41A80000U, 0x422FF596U, 0x4233F905U, 0x4237F9EFU,
-    0x4233F9E7U, 0x423BFD5FU, 0x4237FD56U, 0x423FFE48U, 0x423BFE40U, 0x42380000U,
-    0x42440075U, 0x42400000U, 0x423C0000U, 0x42300000U, 0x423FF07AU, 0x4247F3A0U,
-    0x4247F3A0U, 0x4243F390U, 0x4243F390U, 0x42500000U, 0x42500000U, 0x424C0000U,
-    0x424C0000U, 0x424C0000U, 0x42480000U, 0x42480000U, 0x42580000U, 0x42540000U,
-    0x42540000U, 0x42500000U, 0x42500000U, 0x424C0000U, 0x423000B0U, 0x4233FE2CU,
-    0x4233FE2CU, 0x423805C9U, 0x423405C6U, 0x41DFF5B8U, 0x41E7FBB6U, 0x41EFFF8AU,
-    0x41E7FF7AU, 0x41F80524U, 0x41F00514U, 0x41F00000U, 0x42000000U, 0x42040000U,
-    0x41804030U, 0x41884710U, 0x41884C86U, 0x4190539AU, 0x41FFEB3DU, 0x4207F9EFU,
-    0x4203F9E7U, 0x42100000U, 0x420C0000U, 0x42080000U, 0x41EFFBC9U, 0x41F7F6C2U,
-    0x42000249U, 0x427FD895U, 0x4283EA6EU, 0x4237F8EEU, 0x41883660U, 0x41903060U,
-    0x427BFB92U, 0x41880B39U, 0x419811E4U, 0x41900916U, 0x420FE80BU, 0x4217E4A3U,
-    0x429FDA3DU, 0x42A3D93EU, 0x42FFD335U, 0x424BDAE6U, 0x4253D768U, 0x426FDE15U,
-    0x4277DACFU, 0x42733E3AU, 0x42733DDCU, 0x42773E53U, 0x41F01590U, 0x41F81B6EU,
-    0x42000ED9U, 0x424FE2DFU, 0x4257DFA1U, 0x41E00C96U, 0x41D8164AU, 0x41E01D2BU,
-    0x41E0103DU, 0x4247FB11U, 0x424FF796U, 0x42080167U, 0x41D01007U, 0x41D81B6EU,
-    0x41F01809U, 0x420800A2U, 0x4283FBE1U, 0x4211F988U, 0x420FE14BU, 0x420BF9B9U,
-    0x42380167U, 0x4203FD94U, 0x417FEB3DU, 0x42C1E9D8U, 0x42C5E845U, 0x41EFFBC9U,
-    0x42BFD792U, 0x42C3D6ADU, 0x41E01007U, 0x41E81B6EU, 0x42001AD7U, 0x42BBE1E7U,
-    0x42BFE10DU, 0x42241AACU, 0x42AFFCA7U, 0x4289F2A5U,
+#include "wiso_table.inc"
 };
 
 float mo_wiso(int iso) { return (iso >= 1 && iso <= 124) ? f_of_bits(WISO_BITS[iso - 1]) : 0.f; }

@@ -63,7 +63,7 @@ def lib():
         L.mo_chi_r32.argtypes = [C.c_int, C.c_double, C.c_int, C.c_float]
         L.mo_chi_f64.restype = C.c_double
         L.mo_chi_f64.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double]
-        spec_args = [C.c_void_p, C.c_double] + [C.c_float] * 6 + [C.c_int32] + [C.c_float] * 4 + [dp]
+        spec_args = [C.c_void_p, C.c_double] + [C.c_float] * 5 + [C.c_int32] + [C.c_float] * 4 + [dp]
         L.mo_spec_r32.argtypes = spec_args
         L.mo_spec_f64.argtypes = spec_args
         L.mo_cascade_r32.argtypes = [fp, fp]
