@@ -9,9 +9,10 @@
 //   double  (MARFA_FP64)  every operation in FP64;
 //   float   (MARFA_FP32K) samples in FP32.  The region of the Humlicek-Kuntz split is
 //           decided on the fast path only when the FP32 estimate of VX is clearly in the
-//           Lorentz region; every other sample decides its region from the FP64 value of
-//           VX = |sqrt(ln2) X / gD| formed as the FP64 form forms it, so both forms pick
-//           the same approximation for the same sample.
+//           Lorentz region; every other sample decides its region from the FP64 value
+//           VX = |X| sqrt(ln2)/gD (2 ulp from the oracle's |sqrt(ln2) X / gD|), so both
+//           forms pick the same approximation for the same sample except when VX lies
+//           within ~1e-16 relative of a region boundary.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -53,9 +54,11 @@ __device__ inline R chi_factor(int chi, R ax, R B1, R B2) {
             return R(0.025f) * dexp<R>(-(R(0.009f) * ax));
         }
     } else if (chi == 2) {
+        // exp(-|X|/10) and exp(-|X|/47) with the divisions taken as products by the
+        // reciprocals (one rounding of the exponent's argument apart)
         if (ax >= R(3)) {
-            if (ax < R(10)) return R(1.35f) * dexp<R>(-ax / R(10));
-            return R(0.614f) * dexp<R>(-ax / R(47));
+            if (ax < R(10)) return R(1.35f) * dexp<R>(-ax * R(0.1));
+            return R(0.614f) * dexp<R>(-ax * R(1.0 / 47.0));
         }
     } else if (chi == 3) {
         const R B3 = R(0.0232f);
@@ -80,7 +83,9 @@ struct LineShape {
     R dopa;   // sqrt(ln2)/(sqrt(pi) gD) S    : Doppler amplitude
     R igd;    // 1/gD
     R gL, S;
-    double gD64, gL64;  // region decisions (FP64, as the FP64 form forms them)
+    // region decisions in FP64: VX = |X| xs64 (within 2 ulp of the FP64 form's
+    // |sqrt(ln2) X / gD|), VY = vy64 formed as the FP64 form forms it
+    double xs64, vy64;
 };
 
 template <typename R>
@@ -109,8 +114,8 @@ __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
         p.gL = l;
         p.S = s;
     }
-    p.gD64 = gD;
-    p.gL64 = gL;
+    p.xs64 = sq / gD;
+    p.vy64 = sq * gL / gD;
     return p;
 }
 
@@ -126,6 +131,25 @@ __device__ inline R chi_of(const ShapeCtl &c, R ax) {
     if constexpr (sizeof(R) == 8) return chi_factor<double>(c.chi_id, ax, c.B1d, c.B2d);
     else return chi_factor<float>(c.chi_id, ax, c.B1, c.B2);
 }
+
+// chi factor chosen at compile time (CHI >= 0), or at run time from the control (CHI < 0)
+template <typename R, int CHI>
+__device__ inline R chi_sel(const ShapeCtl &c, R ax) {
+    if constexpr (CHI < 0) {
+        return chi_of<R>(c, ax);
+    } else if constexpr (CHI == 0) {
+        return R(1);
+    } else {
+        if (c.mol != 2) return R(1);
+        if constexpr (sizeof(R) == 8) return chi_factor<double>(CHI, ax, c.B1d, c.B2d);
+        else return chi_factor<float>(CHI, ax, c.B1, c.B2);
+    }
+}
+
+// 1/x: the FP32 form takes the hardware reciprocal (1 ulp), the FP64 form divides
+template <typename R> __device__ inline R recip(R x);
+template <> __device__ inline double recip<double>(double x) { return 1.0 / x; }
+template <> __device__ inline float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Humlicek-Kuntz K(x,y) rational regions 1-3 (src/Shapes.f90:60-117), VY = y, V2 = x^2
 template <typename R>
@@ -182,10 +206,10 @@ __constant__ static const float ASYM_W[9] = {-0.688f, 0.2667f, 0.6338f, 0.4405f,
                                              0.1601f, 0.1131f, 0.0853f, 0.068f};
 
 // Lorentz x chi (src/Spectroscopy.f90:202,227)
-template <typename R>
+template <typename R, int CHI = -1>
 __device__ inline R chilor(R X, const LineShape<R> &p, const ShapeCtl &c) {
-    R lor = p.lorc / (X * X + p.gl2);
-    if (c.chi_id != 0 && c.mol == 2) lor *= chi_of<R>(c, X < R(0) ? -X : X);
+    R lor = p.lorc * recip<R>(X * X + p.gl2);
+    if constexpr (CHI != 0) lor *= chi_sel<R, CHI>(c, X < R(0) ? -X : X);
     return lor;
 }
 
@@ -199,9 +223,8 @@ __device__ inline R doppler(R X, const LineShape<R> &p) {
 // the FP64 form forms them (src/Shapes.f90:49-50), the value in precision R.
 template <typename R>
 __device__ inline R voigt_regions(double X64, const LineShape<R> &p, const ShapeCtl &c) {
-    const double sq = (double)MARFA_SQLN2_F;
-    const double VX = fabs((sq * X64) / p.gD64);
-    const double VY = sq * p.gL64 / p.gD64;
+    const double VX = fabs(X64) * p.xs64;
+    const double VY = p.vy64;
     const R X = (R)X64;
     if (VX >= 15.0) return chilor<R>(X, p, c);
     int region;
@@ -218,9 +241,8 @@ __device__ inline R voigt_regions(double X64, const LineShape<R> &p, const Shape
         return chilor<R>(X, p, c) * (R(1) + R(1.5f) / (vx * vx));
     }
     if (VX > (double)MARFA_SQRT14_F) {
-        int I;
-        if constexpr (sizeof(R) == 8) I = (int)(VX / 0.5 - (double)1.00001f);
-        else I = (int)((float)VX / 0.5f - 1.00001f);
+        // table index decided in FP64 as the FP64 form decides it
+        const int I = (int)(VX / 0.5 - (double)1.00001f);
         R vx = (R)VX;
         R F = R(2) * (R(ASYM_W[I - 1]) * (R(ASYM_U[I]) - vx) + R(ASYM_W[I]) * (vx - R(ASYM_U[I - 1])));
         return doppler<R>(X, p) + p.lorc / (X * X) * (R(1) + F);
@@ -237,14 +259,14 @@ __device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const Sh
 // Profile value at signed distance X64 (FP64) from the shifted centre.  NEAR: the caller
 // samples close to line centres, where the Humlicek-Kuntz regions are the common case,
 // so they are inlined.
-template <typename R, bool NEAR = false>
+template <typename R, bool NEAR = false, int CHI = -1>
 __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl &c) {
     const R X = (R)X64;
-    if (c.shape_id == 1) return chilor<R>(X, p, c);  // lorentzShape
+    if (c.shape_id == 1) return chilor<R, CHI>(X, p, c);  // lorentzShape
     const R ax = X < R(0) ? -X : X;
     // fast path: clearly in region 0 (pure Lorentz x chi); margin covers the FP32 error of VX
     const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
-    if (ax * p.xs >= margin) return chilor<R>(X, p, c);
+    if (ax * p.xs >= margin) return chilor<R, CHI>(X, p, c);
     if constexpr (NEAR) return voigt_regions<R>(X64, p, c);
     else return voigt_slow<R>(X64, p, c);
 }

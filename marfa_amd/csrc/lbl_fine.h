@@ -249,23 +249,18 @@ __device__ inline void gather_level_node(const Src &src, int k, int q, bool need
 template <typename R, typename Src>
 __device__ inline void gather_fine_node(const Src &src, int i, bool left_tile, bool right_tile, Acc<R> &acc) {
     const double xi = (double)(i - 1) * HF;
-    {   // centre lines, left part of the core: u in [(i-1)H, (floor((i-2)/4) + 2) H9)
+    {   // centre lines whose core reaches point i.  Left part (the line at or right of the
+        // point): u in [(i-1)H, (floor((i-2)/4) + 2) H9).  Right part (the line left of the
+        // point): u < (i-1)H and v = 10 - u < (floor((NT-i)/4) + 2) H9.  Both sides are
+        // contiguous in the sorted order, so one candidate range serves them.
         const double uhi = (double)(((i - 2) >> 2) + 2) * H9;
-        int j0, j1;
-        src.range(xi, uhi, j0, j1);
-        for (int j = j0; j < j1; j++) {
-            double u;
-            if (src.cls(j, u) != 2 || u < xi || !(u < uhi)) continue;
-            acc.add(shape_eval<R, true>(u - xi, src.shape(j), src.ctl));
-        }
-    }
-    {   // right part: u < (i-1)H and v < (floor((NT-i)/4) + 2) H9
         const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
         int j0, j1;
-        src.range(10.0 - vhi - 1e-9, xi, j0, j1);
+        src.range(10.0 - vhi - 1e-9, uhi, j0, j1);
         for (int j = j0; j < j1; j++) {
             double u;
-            if (src.cls(j, u) != 2 || !(u < xi) || !(10.0 - u < vhi)) continue;
+            if (src.cls(j, u) != 2) continue;
+            if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) continue;
             acc.add(shape_eval<R, true>(u - xi, src.shape(j), src.ctl));
         }
     }

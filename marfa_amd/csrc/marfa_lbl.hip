@@ -209,7 +209,7 @@ __device__ inline bool level0_masks(double nus, double org, double cut, uint32_t
     return true;
 }
 
-template <typename R>
+template <typename R, int CHI>
 __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
     const int nitems = a.nlev * a.nrec;
     const int r = xcd_item(nitems);
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
         for (int q = 0; q <= 20; q++) {
             // positions no role needs are moved far out, onto the cheap Lorentz branch
             const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
-            const R val = shape_eval<R>(X, p, ctl);
+            const R val = shape_eval<R, false, CHI>(X, p, ctl);
             if (q == 0) aR.add(val);
             if (q & 1) {
                 aM[q >> 1].add(((mM >> q) & 1u) ? val : R(0));
@@ -602,10 +602,24 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         const int nitems = nb * nrec;
         const unsigned gcoarse = (unsigned)(8 * ((nitems + 7) / 8));
         const unsigned gfine = (unsigned)(8 * ((nitems * TILES + 7) / 8));
-        if (job.precision == MARFA_FP32K)
-            hipLaunchKernelGGL(coarse_kernel<float>, dim3(gcoarse), dim3(BLOCK), 0, st, ka);
-        else
-            hipLaunchKernelGGL(coarse_kernel<double>, dim3(gcoarse), dim3(BLOCK), 0, st, ka);
+        // the chi factor is a compile-time choice of the level-0 kernel (as in the reference,
+        // src/ChiFactors.f90:11-16), so its hot loop carries only the selected branch
+        {
+            const bool f32 = job.precision == MARFA_FP32K;
+            const int chi = job.mol_code == 2 ? job.chi_id : 0;
+#define MARFA_COARSE(CH)                                                                              \
+    do {                                                                                              \
+        if (f32) hipLaunchKernelGGL((coarse_kernel<float, CH>), dim3(gcoarse), dim3(BLOCK), 0, st, ka);  \
+        else hipLaunchKernelGGL((coarse_kernel<double, CH>), dim3(gcoarse), dim3(BLOCK), 0, st, ka);     \
+    } while (0)
+            switch (chi) {
+            case 1: MARFA_COARSE(1); break;
+            case 2: MARFA_COARSE(2); break;
+            case 3: MARFA_COARSE(3); break;
+            default: MARFA_COARSE(0); break;
+            }
+#undef MARFA_COARSE
+        }
         HIPCHK(hipGetLastError());
         if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
         if (job.precision == MARFA_FP32K)
