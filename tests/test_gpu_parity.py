@@ -100,3 +100,49 @@ def test_determinism(ctx, tips):
         a = ctx.pt_tables(lbl.JobSpec.from_job(job, prec))
         b = ctx.pt_tables(lbl.JobSpec.from_job(job, prec))
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["aligned", "offset", "start_below_cutoff", "wide_cut", "past_lines"])
+def test_windows_match_oracle(ctx, tips, case):
+    """Subinterval windows and record numbers (app/main.f90:116-144, :196-248)."""
+    from marfa_amd import lbl
+    job = {"aligned": small_job(),
+           "offset": small_job(a=4003.5, b=4041.25),
+           "start_below_cutoff": small_job(lo=5.0, hi=80.0, a=10.0, b=60.0, cut=25.0),
+           "wide_cut": small_job(lo=3700.0, hi=4400.0, a=4000.0, b=4100.0, cut=250.0),
+           "past_lines": small_job(lo=3985.0, hi=4020.0, a=4000.0, b=4060.0)}[case]
+    ctx.load_job(job, tips)
+    oj = po.OracleJob(job, tips)
+    lo, hi, rec, org = ctx.windows(lbl.JobSpec.from_job(job, lbl.FP64))
+    olo, ohi, orec, oorg = oj.windows()
+    assert np.array_equal(lo, olo) and np.array_equal(hi, ohi)
+    assert np.array_equal(rec, orec) and np.array_equal(org, oorg)
+
+
+def test_lbl_scheme_drop_in(ctx, tips):
+    """marfa_lbl_scheme called the way app/main.f90:116-160 calls lblScheme: the carried
+    line index and each subinterval's record match the windows and the FP64 oracle."""
+    from marfa_amd import lbl
+    job = small_job(seed=13, n=2500, lo=3960.0, hi=4070.0, a=4000.0, b=4040.0)
+    ctx.load_job(job, tips)
+    oj = po.OracleJob(job, tips)
+    ref, _ = oj.pt_tables(po.F64)
+    lo, hi, _, _ = ctx.windows(lbl.JobSpec.from_job(job, lbl.FP64))
+    nu = np.asarray(job.lines.nu, np.float64)
+    cut = np.float32(job.cutoff)
+    start = 1
+    if job.start_wv > cut:
+        start = int(np.searchsorted(nu, job.start_wv - cut, side="right")) + 1
+    start_line_wv = nu[start - 1]
+    pself = synth.p_self(job.atm)
+    for lev in range(len(job.atm)):
+        idx, a, r = start, float(job.start_wv), 0
+        while a < job.end_wv:
+            print(f"lbl_scheme level {lev} subinterval {r}", flush=True)
+            assert idx - 1 == lo[r]
+            ext_start = a - float(cut)
+            cap = max(ext_start + 10.0, start_line_wv)
+            idx, rk = ctx.lbl_scheme(idx, cap, a, a + 10.0 + float(cut), float(cut),
+                                     float(job.atm.P[lev]), float(job.atm.T[lev]), float(pself[lev]))
+            assert rel_err(rk, ref[lev, r]).max() < 1e-6
+            a, r = a + 10.0, r + 1
