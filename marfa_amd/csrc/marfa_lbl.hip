@@ -854,7 +854,7 @@ int marfa_lbl_scheme(marfa_ctx *c, int64_t *line_idx, double cap_wv, double star
 }
 
 int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
-    // samples[0]: distinct samples of the reference geometry; samples[1]: coarse_kernel evaluations
+    // samples[0]: distinct samples of the reference geometry; samples[1]: their grid-0 part
     int rc = check_job(c, job);
     if (rc) return rc;
     if (!samples || c->hP.empty()) return set_err(c, MARFA_E_ARG, "levels and output required");
