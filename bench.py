@@ -7,8 +7,8 @@ chi factor, FP32 samples with Kahan-compensated sums and FP64 offsets.  One step
 PT-table record of every level (60 x 950 records x 20,481 points), inputs resident in HBM,
 records left in HBM.
 
-Printed JSON line: value = PT-table points/s; roofline of the dominant kernel (the
-level-0 deposit) from HIP-event timings of the timed steps; accuracy vs the FP64 oracle on
+Printed JSON line: value = PT-table points/s; roofline of the dominant kernel (the deposit
+kernel with the largest HIP-event time over the timed steps); accuracy vs the FP64 oracle on
 sampled records; cpu_baseline = the reference's own Fortran LBL modules (built from its
 sources into oracle/_ref) timed on a bounded sample of the same workload on this host.
 """
@@ -56,36 +56,43 @@ def accuracy(job, tips, gpu_records, picks):
             "records_checked": len(picks), "vs": "FP64 gridded oracle (oracle/marfa_oracle.c, MO_F64)"}
 
 
-def cpu_baseline(job, ctx_counts_full, records=200, level=0):
-    """Time the reference's LBL modules (oracle/_ref/ref_driver, one thread) on `records`
-    subintervals of one level, and scale to the full workload by profile samples."""
+def cpu_baseline(job, ctx_counts_full, target_samples=1.0e9, level=0):
+    """Time the reference's LBL modules (oracle/_ref/ref_driver, one thread) on the first
+    subintervals of one level, as many as make about `target_samples` profile samples
+    (~15-20 s of CPU), and scale to the full workload by profile samples."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
     from marfa_amd import synth, lbl
     if not refrun.available():
         return None
-    a = float(job.start_wv)
-    b = min(float(job.end_wv), a + 10.0 * records)
     atm = synth.Atmosphere(P=job.atm.P[level:level + 1], T=job.atm.T[level:level + 1],
                            rho=job.atm.rho[level:level + 1], h=job.atm.h[level:level + 1])
-    sub = synth.Job(job.lines, atm, a, b, job.cutoff, mol_code=job.mol_code, chi_id=job.chi_id,
-                    shape_id=job.shape_id, target=job.target)
+    a = float(job.start_wv)
+    tips = synth.load_tips()
+    ctx = lbl.LBLContext(0)
+    records, s_sub = 25, 0
+    while True:
+        b = min(float(job.end_wv), a + 10.0 * records)
+        sub = synth.Job(job.lines, atm, a, b, job.cutoff, mol_code=job.mol_code, chi_id=job.chi_id,
+                        shape_id=job.shape_id, target=job.target)
+        # samples of the sample (GPU counter, pinned against the oracle's counters in tests)
+        ctx.load_job(sub, tips)
+        s_sub = ctx.count_samples(lbl.JobSpec.from_job(sub))
+        if s_sub >= target_samples or b >= float(job.end_wv):
+            break
+        records = int(records * min(4.0, max(1.25, 1.1 * target_samples / max(s_sub, 1))))
+    ctx.close()
     recs, wall, out = refrun.run(sub, use_driver=True)
     lbl_s = None
     for line in out.splitlines():
         if line.startswith("LBL_SECONDS"):
             lbl_s = float(line.split()[1])
-    # samples of the sample (GPU counter, pinned against the oracle's counters in tests)
-    ctx = lbl.LBLContext(0)
-    ctx.load_job(sub, synth.load_tips())
-    s_sub = ctx.count_samples(lbl.JobSpec.from_job(sub))
-    ctx.close()
     pts_sub = recs.shape[0] * recs.shape[1] * lbl.NT
     t = lbl_s if lbl_s else wall
     samples_full, points_full = ctx_counts_full
     sps = s_sub / t
     return {"value": sps * points_full / samples_full, "unit": "pts/s", "cores": 1, "kind": "reference",
-            "sample": (f"reference LBL modules (src/*.f90, amdflang -O2, chi=pollack) via oracle/ref_driver: "
+            "sample": (f"reference LBL modules (src/*.f90, amdflang -O2, chi id {job.chi_id}) via oracle/ref_driver: "
                        f"level {level + 1}, {recs.shape[1]} records {a:.0f}-{b:.0f} cm-1, {t:.1f} s in the "
                        f"level/subinterval loops ({wall:.1f} s wall); {sps:.3e} samples/s scaled to the full "
                        f"workload by its samples per point"),
@@ -101,7 +108,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 drop-in timing")
-    ap.add_argument("--cpu-records", type=int, default=150)
+    ap.add_argument("--cpu-samples", type=float, default=1.0e9,
+                    help="profile samples in the reference CPU sample (~17 s at 5.8e7 samples/s)")
     ap.add_argument("--nlines", type=int, default=2_000_000)
     ap.add_argument("--nlev", type=int, default=60)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -226,7 +234,7 @@ def main():
 
     if not args.no_cpu_baseline:
         t3 = time.time()
-        result["cpu_baseline"] = cpu_baseline(job, (samples, points), records=args.cpu_records)
+        result["cpu_baseline"] = cpu_baseline(job, (samples, points), target_samples=args.cpu_samples)
         log(f"[bench] cpu baseline {result['cpu_baseline']} ({time.time() - t3:.1f}s)")
         if result["cpu_baseline"]:
             result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]

@@ -258,7 +258,8 @@ __device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const Sh
 
 // Profile value at signed distance X64 (FP64) from the shifted centre.  NEAR: the caller
 // samples close to line centres, where the Humlicek-Kuntz regions are the common case,
-// so they are inlined.
+// so the FP32 form inlines them (the FP64 form keeps them out of line: inlined FP64
+// regions double the fine kernel's registers and cut its occupancy to one wave per SIMD).
 template <typename R, bool NEAR = false, int CHI = -1>
 __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl &c) {
     const R X = (R)X64;
@@ -267,7 +268,7 @@ __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl
     // fast path: clearly in region 0 (pure Lorentz x chi); margin covers the FP32 error of VX
     const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
     if (ax * p.xs >= margin) return chilor<R, CHI>(X, p, c);
-    if constexpr (NEAR) return voigt_regions<R>(X64, p, c);
+    if constexpr (NEAR && sizeof(R) == 4) return voigt_regions<R>(X64, p, c);
     else return voigt_slow<R>(X64, p, c);
 }
 

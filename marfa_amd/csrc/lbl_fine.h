@@ -7,7 +7,9 @@
 // scheduling and no atomics are used.  The lines depositing into a node are found in
 // O(1): a per-level index of the sorted shifted centres in 1/512 cm-1 buckets gives the
 // candidate range, and the reference's walk is inverted in closed form for the exact
-// test.  For a line at u = nu* - a (src/LineGridCalc.f90:191) the left walk of centerLBL
+// test.  Grids 5..9 and the fine points read their centre lines from an LDS copy of the
+// tile's reach (u and sorted index, with a local bucket table); grids 1..4, whose reach
+// spans several tiles, and the wing lines read the level's sorted arrays directly.  For a line at u = nu* - a (src/LineGridCalc.f90:191) the left walk of centerLBL
 // (:201-331) covers, on grid k, the cells (2 e_{k-1}, e_k] with e_k = max(0, floor(u 2^k) - 1);
 // the right walk (:343-484) the mirror image in v = 10 - u; the core fine points are
 // [4 e_9 + 2, floor(u/H) + 1] (:332-339) and [floor(u/H) + 2, NT - 4 e'_9] (:485-490).
@@ -118,37 +120,45 @@ struct GlobalSrc {
     __device__ inline const LineShape<R> &shape(int j) const { return f.shp[j]; }
 };
 
-// Candidate lines of the tile staged in LDS: u and class per line, and the slice of the
-// bucket index covering the staged window (indices relative to the first staged line).
+// Centre lines (class 2) of one tile's fine reach, staged in LDS in sorted order: u = nu* - org
+// and the sorted index, plus a local bucket table s_b[b] = first staged line with
+// u >= ustart + b/512.  Grids 5..9 and the fine points of a tile [t, t+1) cm-1 only see
+// centre lines with u in [t - 4 H_5, t + 1 + 4 H_5) (left walk of grid k: u in
+// [x + H_k, x + 4 H_k); right walk mirrored; fine core within 2 H_9 of the point), so
+// the staged reach [t - STAGE_REACH, t + 1 + STAGE_REACH) holds every line they need.
+constexpr double STAGE_REACH = 0.1328125;  // 4 H_5 + 4/512, exact in binary
+constexpr int STAGE_NB = 648;              // (1 + 2 STAGE_REACH) * 512 buckets
+constexpr int STAGE_CAP = 1024;            // staged lines; denser tiles read global memory
+
 template <typename R>
-struct LdsSrc {
+struct CentreLds {
     const double *su;
-    const uint8_t *scls;
-    const int *sb;
-    int B0, nsb;
-    double org, base;
+    const int32_t *sj;
+    const int32_t *sb;
+    double ustart;
     const LineShape<R> *shp;
     ShapeCtl ctl;
     __device__ inline void range(double u0, double u1, int &j0, int &j1) const {
-        const double x0 = (org + u0 - base) * BUCKETS_PER_CM, x1 = (org + u1 - base) * BUCKETS_PER_CM;
-        const int b0 = (int)fmin(fmax(floor(x0) - 1.0 - B0, 0.0), (double)nsb);
-        const int b1 = (int)fmin(fmax(floor(x1) + 2.0 - B0, 0.0), (double)nsb);
+        const double x0 = (u0 - ustart) * BUCKETS_PER_CM, x1 = (u1 - ustart) * BUCKETS_PER_CM;
+        const int b0 = (int)fmin(fmax(floor(x0) - 1.0, 0.0), (double)STAGE_NB);
+        const int b1 = (int)fmin(fmax(floor(x1) + 2.0, 0.0), (double)STAGE_NB);
         j0 = sb[b0];
         j1 = b1 > b0 ? sb[b1] : j0;
     }
     __device__ inline int cls(int j, double &u) const {
         u = su[j];
-        return scls[j];
+        return 2;
     }
-    __device__ inline const LineShape<R> &shape(int j) const { return shp[j]; }
+    __device__ inline const LineShape<R> &shape(int j) const { return shp[sj[j]]; }
 };
 
 // Sums at one position node of grid k: global position q in steps of H_k/2.  Even q is the
 // right edge (L) of cell q/2 -> aL and the left edge (P) of cell q/2+1 -> aP; odd q is the
 // midpoint of cell (q+1)/2 -> aL.  needL / needP: the roles this tile owns.
+// Centre lines first (left walk, then right walk), then wing lines, each in sorted order.
 template <typename R, typename Src>
-__device__ inline void gather_level_node(const Src &src, int k, int q, bool needL, bool needP,
-                                         int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+__device__ inline void gather_level_centre(const Src &src, int k, int q, bool needL, bool needP,
+                                           int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
     const int NTk = 10 << k;
     const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
     const bool odd = q & 1;
@@ -193,6 +203,13 @@ __device__ inline void gather_level_node(const Src &src, int k, int q, bool need
             if (toP) aP.add(val);
         }
     }
+}
+
+template <typename R, typename Src>
+__device__ inline void gather_level_wings(const Src &src, int k, int q, bool needL, bool needP,
+                                          int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+    const int NTk = 10 << k;
+    const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
     // wing lines closer than H_{k-1} to the left edge: cells 1 (m == k) and 2 (m >= k)
     if (q <= 4) {
         const double hkm1 = hk(k - 1);
@@ -245,9 +262,16 @@ __device__ inline void gather_level_node(const Src &src, int k, int q, bool need
     }
 }
 
-// direct deposits at fine point i (1-based record index, position (i-1) H)
 template <typename R, typename Src>
-__device__ inline void gather_fine_node(const Src &src, int i, bool left_tile, bool right_tile, Acc<R> &acc) {
+__device__ inline void gather_level_node(const Src &src, int k, int q, bool needL, bool needP,
+                                         int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+    gather_level_centre<R>(src, k, q, needL, needP, jfirst, jstep, aL, aP);
+    gather_level_wings<R>(src, k, q, needL, needP, jfirst, jstep, aL, aP);
+}
+
+// direct deposits at fine point i (1-based record index, position (i-1) H): centre lines
+template <typename R, typename Src>
+__device__ inline void gather_fine_centre(const Src &src, int i, Acc<R> &acc) {
     const double xi = (double)(i - 1) * HF;
     {   // centre lines whose core reaches point i.  Left part (the line at or right of the
         // point): u in [(i-1)H, (floor((i-2)/4) + 2) H9).  Right part (the line left of the
@@ -264,7 +288,12 @@ __device__ inline void gather_fine_node(const Src &src, int i, bool left_tile, b
             acc.add(shape_eval<R, true>(u - xi, src.shape(j), src.ctl));
         }
     }
-    if (left_tile && i <= 5) {  // leftLBL :107-112 (d < H9)
+}
+
+// direct deposits of wing lines at the first / last fine points of the record
+template <typename R, typename Src>
+__device__ inline void gather_fine_wings(const Src &src, int i, Acc<R> &acc) {
+    if (i <= 5) {  // leftLBL :107-112 (d < H9)
         int j0, j1;
         src.range(-H9, 0.0, j0, j1);
         for (int j = j0; j < j1; j++) {
@@ -274,7 +303,7 @@ __device__ inline void gather_fine_node(const Src &src, int i, bool left_tile, b
             acc.add(shape_eval<R, true>(X, src.shape(j), src.ctl));
         }
     }
-    if (right_tile && i >= NT - 3) {  // rightLBL :592-595 (d < H9)
+    if (i >= NT - 3) {  // rightLBL :592-595 (d < H9)
         int j0, j1;
         src.range(10.0 - 1e-9, 10.0 + H9 + 1e-9, j0, j1);
         for (int j = j0; j < j1; j++) {
@@ -292,7 +321,6 @@ __device__ inline void gather_fine_node(const Src &src, int i, bool left_tile, b
 __device__ inline int lvl_node_off(int k) { return (1 << (k + 1)) - 4 + (k - 1); }
 // offset of grid k's P block in the tile's node sums; mid at +2^k, L at +2*2^k
 __device__ inline int lvl_acc_off(int k) { return 3 * ((1 << k) - 2); }
-constexpr int FINE_OFF = 3066;
 constexpr int WAVE_NODES = 64;  // position nodes of grids 1..4
 
 __device__ inline void store_level_node(double *g, int k, int t, int qloc, double vL, double vP) {
@@ -307,7 +335,11 @@ __device__ inline void store_level_node(double *g, int k, int t, int qloc, doubl
 
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
-    __shared__ double g[NACC];
+    __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
+    __shared__ double s_u[STAGE_CAP];     // staged centre lines
+    __shared__ int32_t s_j[STAGE_CAP];
+    __shared__ int32_t s_b[STAGE_NB + 1];
+    __shared__ int s_cnt[BLOCK / 64];
     const int nitems = a.nlev * a.nrec * TILES;
     const int w = xcd_item(nitems);
     if (w >= nitems) return;
@@ -328,13 +360,59 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     f.whi = a.hi[s];
     f.ctl = make_ctl(a, a.lev0 + lev);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-
-    // Candidates are read from the level's sorted arrays through the bucket index (L2-resident
-    // for neighbouring tiles).  Staging them in LDS (LdsSrc) measured slower on config 2: the
-    // extra 23 KB of LDS halves the resident waves of this latency-bound gather.
     const GlobalSrc<R> gsrc(f);
 
-    // grids 1..4: one wavefront per node, lines split over the lanes, fixed-order reduction
+    // ---- stage the centre lines of the tile's fine reach: order-preserving compaction of
+    // the sorted candidates (one ballot per wave, wave counts through LDS)
+    const double ustart = (double)t - STAGE_REACH, uend = (double)t + 1.0 + STAGE_REACH;
+    int nst = 0;
+    {
+        int J0, J1;
+        cand(f, ustart, uend, J0, J1);
+        for (int base = J0; base < J1; base += BLOCK) {
+            const int j = base + tid;
+            double u = 0.0;
+            const bool keep = j < J1 && classify(f, j, u) == 2 && u >= ustart && u < uend;
+            const uint64_t m = __ballot(keep);
+            if (lane == 0) s_cnt[wv] = __popcll(m);
+            __syncthreads();
+            int off = nst, tot = 0;
+            for (int w2 = 0; w2 < BLOCK / 64; w2++) {
+                if (w2 < wv) off += s_cnt[w2];
+                tot += s_cnt[w2];
+            }
+            off += __popcll(m & ((1ull << lane) - 1ull));
+            if (keep && off < STAGE_CAP) {
+                s_u[off] = u;
+                s_j[off] = j;
+            }
+            nst += tot;
+            __syncthreads();
+        }
+    }
+    const bool staged = nst <= STAGE_CAP;  // uniform over the workgroup
+    if (staged)
+        for (int b = tid; b <= STAGE_NB; b += BLOCK) {
+            const double x = ustart + (double)b * (1.0 / BUCKETS_PER_CM);
+            int lo = 0, hi = nst;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_u[mid] < x) lo = mid + 1;
+                else hi = mid;
+            }
+            s_b[b] = lo;
+        }
+    __syncthreads();
+    CentreLds<R> lsrc;
+    lsrc.su = s_u;
+    lsrc.sj = s_j;
+    lsrc.sb = s_b;
+    lsrc.ustart = ustart;
+    lsrc.shp = f.shp;
+    lsrc.ctl = f.ctl;
+
+    // ---- grids 1..4: one wavefront per node, lines split over the lanes, fixed-order
+    // reduction (their reach spans several tiles: candidates come from global memory)
     for (int nd = wv; nd < WAVE_NODES; nd += BLOCK / 64) {
         int k = 1;
         while (k < 4 && nd >= lvl_node_off(k + 1)) k++;
@@ -351,45 +429,35 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
         }
         if (lane == 0) store_level_node(g, k, t, qloc, vL, vP);
     }
-    // grids 5..9 and the fine points: one thread per node
-    for (int nd = WAVE_NODES + tid; nd < NNODES; nd += BLOCK) {
+    // ---- grids 5..9: one thread per node, centre lines from LDS
+    for (int nd = WAVE_NODES + tid; nd < NLVL_NODES; nd += BLOCK) {
+        int k = 5;
+        while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
+        const int qloc = nd - lvl_node_off(k), span = 2 << k, q = t * span + qloc;
+        const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
         Acc<R> aL, aP;
         aL.zero();
         aP.zero();
-        if (nd < NLVL_NODES) {
-            int k = 5;
-            while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
-            const int qloc = nd - lvl_node_off(k), span = 2 << k;
-            const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-            gather_level_node<R>(gsrc, k, t * span + qloc, nL, nP, 0, 1, aL, aP);
-            store_level_node(g, k, t, qloc, aL.value(), aP.value());
-        } else {
-            const int il = nd - NLVL_NODES;
-            gather_fine_node<R>(gsrc, 2048 * t + 2 + il, t == 0, t == TILES - 1, aL);
-            g[FINE_OFF + il] = aL.value();
-        }
+        if (staged) gather_level_centre<R>(lsrc, k, q, nL, nP, 0, 1, aL, aP);
+        else gather_level_centre<R>(gsrc, k, q, nL, nP, 0, 1, aL, aP);
+        gather_level_wings<R>(gsrc, k, q, nL, nP, 0, 1, aL, aP);
+        store_level_node(g, k, t, qloc, aL.value(), aP.value());
     }
     __syncthreads();
 
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
     if (a.grids && r == 0) {
         for (int e = tid; e < NACC; e += BLOCK) {
-            int dst;
-            if (e >= FINE_OFF) {
-                dst = 30690 + 2048 * t + 1 + (e - FINE_OFF);
-            } else {
-                int k = 1;
-                while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
-                const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
-                dst = 30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl;
-            }
-            a.grids[dst] = g[e] / sc;
+            int k = 1;
+            while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
+            const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
+            a.grids[30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl] = g[e] / sc;
         }
         if (tid < 3) a.grids[tid * 10 + t] = cz[1 + 10 * tid + t] / sc;
         if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
     }
 
-    // ---- cascade of this tile in FP64 (src/Grids.f90:80-139)
+    // ---- cascade of this tile in FP64 (src/Grids.f90:80-139), grids 0..9
     auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i) {
         fP[i] += P;
         double A = 0.375 * P + 0.75 * M - 0.125 * L;
@@ -413,24 +481,42 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
             step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c);
         __syncthreads();
     }
-    {
-        const int cb = lvl_acc_off(9);
-        double *rk = g + FINE_OFF;
-        for (int c = tid; c < 512; c += BLOCK) {
-            const double P = g[cb + c], M = g[cb + 512 + c], L = g[cb + 1024 + c];
-            double A = P * 0.375 + M * 0.75 - L * 0.125;
-            if (A < P || A < M) A = 0.5 * (P + M);
-            rk[4 * c] += A;
-            rk[4 * c + 1] += M;
-            A = L * 0.375 + M * 0.75 - P * 0.125;
-            if (A < L || A < M) A = 0.5 * (L + M);
-            rk[4 * c + 2] += A;
-            rk[4 * c + 3] += L;
-        }
-    }
-    __syncthreads();
+
+    // ---- fine points: direct deposits summed in registers (thread owns the 4 points of
+    // grid-9 cells tid and tid + 256), then the last cascade step and the record write
+    const int cb9 = lvl_acc_off(9);
     const double mul = (double)a.outmul[a.lev0 + lev] / sc;
     float *out = a.out + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
-    for (int e = tid; e < 2048; e += BLOCK) out[2048 * t + 1 + e] = (float)(g[FINE_OFF + e] * mul);
+    for (int h = 0; h < 2; h++) {
+        const int c = tid + h * BLOCK;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int e = 0; e < 4; e++) {
+            const int il = 4 * c + e, i = 2048 * t + 2 + il;
+            Acc<R> acc;
+            acc.zero();
+            if (staged) gather_fine_centre<R>(lsrc, i, acc);
+            else gather_fine_centre<R>(gsrc, i, acc);
+            if (i <= 5 || i >= NT - 3) gather_fine_wings<R>(gsrc, i, acc);
+            const double x = acc.value();
+            // register-resident selection (no dynamic indexing of v)
+            v[0] = e == 0 ? x : v[0];
+            v[1] = e == 1 ? x : v[1];
+            v[2] = e == 2 ? x : v[2];
+            v[3] = e == 3 ? x : v[3];
+            if (a.grids && r == 0) a.grids[30690 + 2048 * t + 1 + il] = x / sc;
+        }
+        const double P = g[cb9 + c], M = g[cb9 + 512 + c], L = g[cb9 + 1024 + c];
+        double A = P * 0.375 + M * 0.75 - L * 0.125;
+        if (A < P || A < M) A = 0.5 * (P + M);
+        v[0] += A;
+        v[1] += M;
+        A = L * 0.375 + M * 0.75 - P * 0.125;
+        if (A < L || A < M) A = 0.5 * (L + M);
+        v[2] += A;
+        v[3] += L;
+#pragma unroll
+        for (int e = 0; e < 4; e++) out[2048 * t + 1 + 4 * c + e] = (float)(v[e] * mul);
+    }
     if (t == 0 && tid == 0) out[0] = (float)(cz[0] * mul);
 }

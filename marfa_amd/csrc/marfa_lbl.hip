@@ -41,7 +41,7 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int TILES = 10;      // 1 cm-1 tiles per 10 cm-1 record
 constexpr int NCOARSE = 32;    // RK1, P[10], M[10], L[10], pad
-constexpr int NACC = 5114;     // per tile: 3*(2+4+...+512) grid nodes + 2048 fine points
+constexpr int NACC = 3066;     // per tile: 3*(2+4+...+512) nodes of grids 1..9
 
 enum : uint8_t { CLS_NONE = 0, CLS_LEFT = 1, CLS_CENTER = 2, CLS_RIGHT = 3 };
 
@@ -278,7 +278,6 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
 
 // ------------------------------------------------------------------ 4. fine + cascade
 constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes per tile
-constexpr int NNODES = NLVL_NODES + 2048;
 #include "lbl_fine.h"
 
 // ------------------------------------------------------------------ sample counts
