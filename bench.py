@@ -99,6 +99,53 @@ def cpu_baseline(job, ctx_counts_full, target_samples=1.0e9, level=0):
             "sample_pts_per_s": pts_sub / t, "samples_per_s": sps}
 
 
+def adjoint_bench(steps=3):
+    """Forward + hand-written adjoint of the 200,000-line, 30-level workload (1000-1100 cm-1,
+    cutoff 25, FP32K), loss sum (log k - log k_hat)^2 with k_hat from S and gamma_air
+    perturbed by U(-5%, 5%): ms per level of the forward, of the adjoint, and of both."""
+    import torch
+    from marfa_amd import synth, lbl
+    from marfa_amd.autograd import PTTableModel, PTTables, spectral_loss
+    job = synth.config4()
+    tips = synth.load_tips()
+    model = PTTableModel(job, tips, precision=lbl.FP32K)
+    rng = np.random.default_rng(synth.SEED_BASE + 4)
+    base = list(model.params())
+    pert = [b.clone() for b in base]
+    for k in (0, 1):
+        pert[k] = pert[k] * torch.as_tensor(1 + rng.uniform(-0.05, 0.05, tuple(pert[k].shape)),
+                                            dtype=torch.float32, device="cuda")
+    model.set_params(*pert)
+    target = model.records()
+    fwd, adj, kf, ka = [], [], [], []
+    for it in range(steps + 1):
+        tp = [b.clone().requires_grad_(True) for b in base]
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        rec = PTTables.apply(*tp, model)
+        e1.record()
+        tf = model.ctx.timings()["total"]
+        loss = spectral_loss(rec, target)
+        loss.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if it > 0:  # first pass warms up
+            fwd.append(e0.elapsed_time(e1))
+            adj.append(e1.elapsed_time(e2))
+            kf.append(tf)
+            ka.append(model.ctx.timings()["total"])
+    nlev = len(job.atm)
+    model.close()
+    f, a = float(np.mean(fwd)), float(np.mean(adj))
+    return {"workload": "200,000 lines 1000-1100 cm-1, 30 levels P 10->1e-3 atm T 300->180 K, cutoff 25, FP32K",
+            "fwd_ms": f, "adjoint_ms": a, "fwd_plus_adjoint_ms_per_level": (f + a) / nlev,
+            "fwd_ms_per_level": f / nlev, "adjoint_ms_per_level": a / nlev, "levels": nlev,
+            "fwd_kernels_ms": float(np.mean(kf)), "adjoint_kernels_ms": float(np.mean(ka)),
+            "note": "fwd_ms / adjoint_ms: stream time of PTTables.apply and of loss.backward (parameter "
+                    "upload, loss gradient in torch, recomputation of the grid sums and the adjoint "
+                    "kernels); *_kernels_ms: HIP-event time of the library's kernels alone"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +155,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 drop-in timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint timing")
     ap.add_argument("--cpu-samples", type=float, default=1.0e9,
                     help="profile samples in the reference CPU sample (~17 s at 5.8e7 samples/s)")
     ap.add_argument("--nlines", type=int, default=2_000_000)
@@ -223,6 +271,10 @@ def main():
         log(f"[bench] FP64 drop-in: {d64 * 1e3:.1f} ms/step")
         ctx.pt_tables_device(spec, out.data_ptr(), stream)  # restore FP32 records for accuracy
         torch.cuda.synchronize()
+
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_bench()
+        log(f"[bench] adjoint {result['adjoint']}")
 
     if not args.no_accuracy:
         rng = np.random.Generator(np.random.PCG64(5))

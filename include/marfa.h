@@ -118,6 +118,22 @@ int marfa_lbl_scheme(marfa_ctx *ctx, int64_t *line_idx, double cap_wv, double st
 int marfa_pt_tables(marfa_ctx *ctx, const marfa_job *job, float *out_dev, float *out_host,
                     void *stream, int32_t async);
 
+/* marfa_pt_tables with FP64 records (no counterpart in the reference, whose records are
+ * FP32): out_dev (device) receives the records before their rounding to FP32, for
+ * precision studies and finite-difference checks of the adjoint.  Synchronous. */
+int marfa_pt_tables_f64(marfa_ctx *ctx, const marfa_job *job, double *out_dev, void *stream);
+
+/* ---- adjoint of marfa_pt_tables (no counterpart in the reference) ----
+ * grad_out: dL/d(record point) on the device, in the layout of marfa_pt_tables' output.
+ * grad_lines (host, [4][n] for the context's n lines): dL/dS, dL/dgamma_air, dL/dn_T,
+ * dL/ddelta; grad_levels (host, [2][nlev]): dL/dT, dL/dP per level (density held fixed,
+ * pSelf = rho 10 k T / p0 follows T).  The deposit geometry (which nodes each line samples)
+ * is held fixed: the derivative is that of the piecewise-smooth model between its
+ * decisions.  The level's grid sums are recomputed (not stored by the forward call).
+ * Not available for the Perrin chi factor (MARFA_E_ARG). */
+int marfa_pt_tables_adjoint(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
+                            double *grad_lines, double *grad_levels);
+
 /* Work counts of the job, summed over levels and subintervals (computed on the device):
  * samples[0] = exact count of distinct profile evaluations (samples) of the reference's
  * grid geometry (21 + 2m per wing line, the walks and core points of a centre line);

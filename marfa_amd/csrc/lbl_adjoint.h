@@ -1,0 +1,231 @@
+// lbl_adjoint.h -- line pass of the adjoint: dL/d(line parameters) and dL/d(level T, P)
+// from dL/d(pre-cascade node) of every record (adjoint_tile_kernel).  Included by
+// marfa_lbl.hip inside its kernel namespace.
+//
+// One thread per (line, level) owns that line's gradient, so no sum needs atomics: it
+// visits every record whose window holds the line, walks the line's samples in the
+// reference's deposit geometry (src/LineGridCalc.f90; the same closed forms as the
+// forward gathers), and accumulates dL/dnode x dF/d(S(T), gD, gL, nu*) in FP64, with F and
+// its derivatives recomputed in the form's precision (shape_grad).  The chain to the
+// parameters follows src/Spectroscopy.f90:38-145 and app/main.f90:92 (pSelf is
+// proportional to T at fixed density).
+#pragma once
+
+struct AdjArgs {
+    const double *nu;                                // [n] unshifted centres (context)
+    const float *S, *gair, *gself, *elow, *nT, *delta;
+    const int32_t *molIso;
+    const float *tips;                               // [74][501]
+    const float *P;                                  // [nlev] absolute level index
+    const double *pself;                             // [nlev]
+    const float *gbar;                               // [batch records][30690 + NT]
+    double *gline;                                   // [batch levels][nl][4]: S, gair, nT, delta
+    double *glev;                                    // [batch levels][blocks][2]: T, P
+};
+
+__device__ inline int node_p(int k, int I) { return 30 * ((1 << k) - 1) + (I - 1); }
+__device__ inline int node_m(int k, int I) { return 30 * ((1 << k) - 1) + (10 << k) + (I - 1); }
+__device__ inline int node_l(int k, int I) { return 30 * ((1 << k) - 1) + 2 * (10 << k) + (I - 1); }
+__device__ inline int node_rk(int i) { return 30690 + i - 1; }
+
+template <typename R>
+struct LineAdj {
+    const float *gr;  // dL/dnode of the record
+    const LineShape<R> *p;
+    const ShapeCtl *ctl;
+    double u;         // nu* - origin
+    double aS, aX, aD, aL;
+    __device__ inline void acc(double x, float gs) {
+        if (gs == 0.f) return;
+        const SampleGrad<R> s = shape_grad<R>(u - x, *p, *ctl);
+        aS += (double)gs * (double)s.v;
+        aX += (double)gs * (double)s.dX;
+        aD += (double)gs * (double)s.dgD;
+        aL += (double)gs * (double)s.dgL;
+    }
+    // positions q = qlo..qhi (steps of H_k/2) of a run of cells of grid k: even q is the L
+    // edge of cell q/2 (q > qlo) and the P edge of cell q/2+1 (q < qhi); odd q a mid point
+    __device__ inline void walk(int k, int qlo, int qhi) {
+        const double h = 0.5 * hk(k);
+        for (int q = qlo; q <= qhi; q++) {
+            float gs;
+            if (q & 1) {
+                gs = gr[node_m(k, (q + 1) >> 1)];
+            } else {
+                gs = 0.f;
+                if (q > qlo) gs += gr[node_l(k, q >> 1)];
+                if (q < qhi) gs += gr[node_p(k, (q >> 1) + 1)];
+            }
+            acc((double)q * h, gs);
+        }
+    }
+    // centerLBL (src/LineGridCalc.f90:181-494): RK(1), the two walks and the core points
+    __device__ inline void centre() {
+        acc(0.0, gr[node_rk(1)]);
+        int ep = 0;
+        for (int k = 0; k <= 9; k++) {
+            const int e = u >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(u, k)) - 1 : 0;
+            if (e > 2 * ep) walk(k, 4 * ep, 2 * e);
+            ep = e;
+        }
+        const int fu = (int)floor(u / HF);
+        if (u >= HF)
+            for (int i = 4 * ep + 2; i <= fu + 1; i++) acc((double)(i - 1) * HF, gr[node_rk(i)]);
+        const double v = 10.0 - u;
+        int er = 0;
+        for (int k = 0; k <= 9; k++) {
+            const int e = v >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(v, k)) - 1 : 0;
+            const int two_nt = 20 << k;
+            if (e > 2 * er) walk(k, two_nt - 2 * e, two_nt - 4 * er);
+            er = e;
+        }
+        for (int i = fu + 2; i <= NT - 4 * er; i++) acc((double)(i - 1) * HF, gr[node_rk(i)]);
+    }
+    // leftLBL (:7-179): the line's right wing from the subinterval's left edge
+    __device__ inline void left() {
+        const double d = -u;
+        acc(0.0, gr[node_rk(1)]);
+        if (d >= 1.0) {
+            walk(0, 0, 20);
+            return;
+        }
+        int m = 1;
+        while (m <= 9 && d < hk(m)) m++;
+        if (m <= 9) {
+            acc(0.0, gr[node_p(m, 1)]);
+            acc(0.5 * hk(m), gr[node_m(m, 1)]);
+            acc(hk(m), gr[node_l(m, 1)]);
+        } else {
+            for (int i = 2; i <= 5; i++) acc((double)(i - 1) * HF, gr[node_rk(i)]);
+        }
+        for (int k = m < 9 ? m : 9; k >= 1; k--) {
+            acc(hk(k), gr[node_p(k, 2)]);
+            acc(1.5 * hk(k), gr[node_m(k, 2)]);
+            acc(2.0 * hk(k), gr[node_l(k, 2)]);
+        }
+        walk(0, 2, 20);
+    }
+    // rightLBL (:496-674): the line's left wing from the subinterval's right edge
+    __device__ inline void right() {
+        const double dr = u - 10.0;
+        acc(0.0, gr[node_rk(1)]);
+        if (dr >= 1.0) {
+            walk(0, 0, 20);
+            return;
+        }
+        int m = 1;
+        while (m <= 9 && dr < hk(m)) m++;
+        if (m <= 9) {
+            const int N = 10 << m;
+            acc(10.0, gr[node_l(m, N)]);
+            acc(10.0 - 0.5 * hk(m), gr[node_m(m, N)]);
+            acc(10.0 - hk(m), gr[node_p(m, N)]);
+        } else {
+            for (int i = NT; i >= NT - 3; i--) acc((double)(i - 1) * HF, gr[node_rk(i)]);
+        }
+        for (int k = m < 9 ? m : 9; k >= 1; k--) {
+            const int N = (10 << k) - 1;
+            acc(10.0 - hk(k), gr[node_l(k, N)]);
+            acc(10.0 - 1.5 * hk(k), gr[node_m(k, N)]);
+            acc(10.0 - 2.0 * hk(k), gr[node_p(k, N)]);
+        }
+        walk(0, 0, 18);
+    }
+};
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b) {
+    const int lev = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double cT = 0.0, cP = 0.0;
+    if (j < a.nl) {
+        const int64_t i = a.lmin + j, o = (int64_t)lev * a.nl + j;
+        const int L = a.lev0 + lev;
+        const double nus = a.nus[o], gD = a.gD[o], gL = a.gL[o], ST = a.S[o];
+        const double sc = a.sscale[L];
+        const LineShape<R> p = make_line<R>(gD, gL, ST * sc);
+        const ShapeCtl ctl = make_ctl(a, L);
+        LineAdj<R> w;
+        w.p = &p;
+        w.ctl = &ctl;
+        w.aS = w.aX = w.aD = w.aL = 0.0;
+        // records whose window [lo, hi) holds line i: hi is non-decreasing in r
+        int r = 0, hiR = a.nrec;
+        while (r < hiR) {
+            const int mid = (r + hiR) >> 1;
+            if (a.hi[mid] > i) hiR = mid;
+            else r = mid + 1;
+        }
+        const double cut = (double)a.cutoff;
+        for (; r < a.nrec && a.lo[r] <= i; r++) {
+            if (i >= a.hi[r]) continue;
+            const double org = a.org[r];
+            w.u = nus - org;
+            w.gr = b.gbar + ((int64_t)lev * a.nrec + r) * (30690 + NT);
+            if (nus < org) {
+                if (!(-w.u > cut)) w.left();
+            } else if (nus > org + 10.0) {
+                if (w.u - 10.0 < cut) w.right();
+            } else if (w.u < 10.0) {
+                w.centre();
+            }
+        }
+        // chain rule to the parameters (src/Spectroscopy.f90:38-145)
+        const double T = (double)a.T[L], P = (double)b.P[L], ps = b.pself[L];
+        const double S0 = (double)b.S[i], ga = (double)b.gair[i], gs = (double)b.gself[i];
+        const double n = (double)b.nT[i], dl = (double)b.delta[i], E = (double)b.elow[i];
+        const double tr = (double)MARFA_TREF_F / T, fT = pow(tr, n);
+        const double dnu = w.aX + w.aD * gD / nus;     // gD is proportional to nu*
+        double *gl = b.gline + o * 4;
+        gl[0] = S0 != 0.0 ? w.aS / S0 : 0.0;           // S(T) is proportional to S
+        gl[1] = w.aL * fT * (P - ps);
+        gl[2] = w.aL * gL * log(tr);
+        gl[3] = dnu * P;                               // nu* = nu + delta P
+        // d ln S(T) / dT: partition sum (linear in the TIPS table), Boltzmann factor and
+        // stimulated emission (unshifted nu)
+        const int iso = b.molIso[i] / 100;
+        const int ti = (int)((T - 20.0) / 2.0 + 1.0);
+        const double wu = (T - ((double)ti * 2.0 + 18.0)) / 2.0;
+        const float *row = b.tips + (iso - 1) * 501;
+        const double Q = (1.0 - wu) * (double)row[ti - 1] + wu * (double)row[ti];
+        const double dQ = ((double)row[ti] - (double)row[ti - 1]) / 2.0;
+        const double c2 = (double)MARFA_C2_F, x = c2 * b.nu[i] / T;
+        const double dlnS = -dQ / Q + c2 * E / (T * T) - (x / T) * exp(-x) / (1.0 - exp(-x));
+        cT = w.aS * dlnS + w.aD * gD / (2.0 * T) + w.aL * (-n * gL / T + fT * (gs - ga) * ps / T);
+        cP = dnu * dl + w.aL * fT * ga;
+    }
+    // per-block partial sums of the level gradients, fixed order
+    for (int off = 32; off > 0; off >>= 1) {
+        cT += __shfl_xor(cT, off);
+        cP += __shfl_xor(cP, off);
+    }
+    __shared__ double red[2][BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = cT;
+        red[1][threadIdx.x >> 6] = cP;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double x = 0.0;
+        for (int w2 = 0; w2 < BLOCK / 64; w2++) x += red[threadIdx.x][w2];
+        b.glev[((int64_t)lev * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = x;
+    }
+}
+
+// running per-line gradient += the batch's levels, in level order
+__global__ __launch_bounds__(BLOCK) void adjoint_reduce_kernel(const double *gline, int nb, int64_t nl, double *gacc) {
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nl) return;
+    double s0 = gacc[j * 4], s1 = gacc[j * 4 + 1], s2 = gacc[j * 4 + 2], s3 = gacc[j * 4 + 3];
+    for (int l = 0; l < nb; l++) {
+        const double *g = gline + ((int64_t)l * nl + j) * 4;
+        s0 += g[0];
+        s1 += g[1];
+        s2 += g[2];
+        s3 += g[3];
+    }
+    gacc[j * 4] = s0;
+    gacc[j * 4 + 1] = s1;
+    gacc[j * 4 + 2] = s2;
+    gacc[j * 4 + 3] = s3;
+}

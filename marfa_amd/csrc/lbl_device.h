@@ -151,7 +151,28 @@ template <typename R> __device__ inline R recip(R x);
 template <> __device__ inline double recip<double>(double x) { return 1.0 / x; }
 template <> __device__ inline float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Humlicek-Kuntz K(x,y) rational regions 1-3 (src/Shapes.f90:60-117), VY = y, V2 = x^2
+// Forward-mode dual number with two tangents (d/dx^2, d/dy): evaluating the Kuntz rationals
+// in it gives their exact partial derivatives for the adjoint, through the same
+// coefficient arithmetic as the value.
+template <typename R>
+struct Dual2 {
+    R v, a, b;
+    __device__ Dual2() {}
+    __device__ Dual2(R x) : v(x), a(R(0)), b(R(0)) {}
+    __device__ Dual2(R x, R da, R db) : v(x), a(da), b(db) {}
+};
+template <typename R> __device__ inline Dual2<R> operator+(Dual2<R> x, Dual2<R> y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
+template <typename R> __device__ inline Dual2<R> operator-(Dual2<R> x, Dual2<R> y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
+template <typename R> __device__ inline Dual2<R> operator*(Dual2<R> x, Dual2<R> y) {
+    return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b};
+}
+template <typename R> __device__ inline Dual2<R> operator/(Dual2<R> x, Dual2<R> y) {
+    const R q = x.v / y.v;
+    return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
+}
+
+// Humlicek-Kuntz K(x,y) rational regions 1-3 (src/Shapes.f90:60-117), VY = y, V2 = x^2.
+// R is float, double or Dual2<float/double> (the coefficients enter as R(constant)).
 template <typename R>
 __device__ inline R kuntz_rational(int region, R VY, R V2) {
     R num, den;
@@ -270,6 +291,108 @@ __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl
     if (ax * p.xs >= margin) return chilor<R, CHI>(X, p, c);
     if constexpr (NEAR && sizeof(R) == 4) return voigt_regions<R>(X64, p, c);
     else return voigt_slow<R>(X64, p, c);
+}
+
+// ---------------------------------------------------------------- derivatives (adjoint)
+// d chi / d|X| of chi_factor: the exponent's slope times chi inside each band, 0 outside.
+template <typename R>
+__device__ inline R dchi_factor(int chi, R ax, R B1, R B2, R chiv) {
+    if (chi == 1) {
+        if (ax > R(3)) return (ax <= R(150) ? R(-0.027f) : ax <= R(300) ? R(-0.016f) : R(-0.009f)) * chiv;
+    } else if (chi == 2) {
+        if (ax >= R(3)) return (ax < R(10) ? R(-0.1) : R(-1.0 / 47.0)) * chiv;
+    } else if (chi == 3) {
+        if (ax > R(3)) return (ax < R(30) ? -B1 : ax < R(120) ? -B2 : R(-0.0232f)) * chiv;
+    }
+    return R(0);
+}
+
+// value and partial derivatives of one sample with respect to the signed offset X (the
+// shifted centre minus the node position), gD and gL; the S(T) derivative is v / S.
+template <typename R>
+struct SampleGrad {
+    R v, dX, dgD, dgL;
+};
+
+template <typename R>
+__device__ inline void chilor_grad(R X, const LineShape<R> &p, const ShapeCtl &c, SampleGrad<R> &o) {
+    const R ax = X < R(0) ? -X : X, sg = X < R(0) ? R(-1) : R(1);
+    const R inv = R(1) / (X * X + p.gl2);
+    const R lor = p.lorc * inv;
+    R chi = R(1), dchi = R(0);
+    if (c.mol == 2 && c.chi_id != 0) {
+        if constexpr (sizeof(R) == 8) {
+            chi = chi_factor<double>(c.chi_id, ax, c.B1d, c.B2d);
+            dchi = dchi_factor<double>(c.chi_id, ax, c.B1d, c.B2d, chi);
+        } else {
+            chi = chi_factor<float>(c.chi_id, ax, c.B1, c.B2);
+            dchi = dchi_factor<float>(c.chi_id, ax, c.B1, c.B2, chi);
+        }
+    }
+    o.v = lor * chi;
+    o.dX = R(-2) * X * lor * inv * chi + lor * dchi * sg;
+    o.dgD = R(0);
+    o.dgL = p.S / R(MARFA_PI_F) * (X * X - p.gl2) * inv * inv * chi;
+}
+
+// Same region decisions as shape_eval / voigt_regions, value and derivatives of the branch
+// taken (src/Shapes.f90:22-133, src/Spectroscopy.f90:194-261).
+template <typename R>
+__device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, const ShapeCtl &c) {
+    SampleGrad<R> o;
+    const R X = (R)X64;
+    const R ax = X < R(0) ? -X : X, sg = X < R(0) ? R(-1) : R(1);
+    const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
+    if (c.shape_id == 1 || ax * p.xs >= margin) {
+        chilor_grad<R>(X, p, c, o);
+        return o;
+    }
+    const double VX = fabs(X64) * p.xs64, VY = p.vy64;
+    if (VX >= 15.0) {
+        chilor_grad<R>(X, p, c, o);
+        return o;
+    }
+    const R vx = (R)VX, vy = (R)VY;
+    int region;
+    if (VX + VY >= 15.0) region = 1;
+    else if (VX + VY >= 5.5) region = 2;
+    else if (VX <= 1.0 || VY >= (double)0.02f) region = 3;
+    else region = 4;
+    if (region < 4) {
+        const Dual2<R> K = kuntz_rational<Dual2<R>>(region, Dual2<R>(vy, R(0), R(1)), Dual2<R>(vx * vx, R(1), R(0)));
+        o.v = K.v * p.normc;
+        o.dX = p.normc * K.a * R(2) * vx * p.xs * sg;
+        o.dgD = -(o.v + p.normc * (R(2) * vx * vx * K.a + vy * K.b)) * p.igd;
+        o.dgL = p.normc * K.b * p.xs;
+        return o;
+    }
+    if (VX > 5.0) {  // asymptote 1: Lorentz x chi x (1 + 1.5 / VX^2)
+        chilor_grad<R>(X, p, c, o);
+        const R f = R(1) + R(1.5f) / (vx * vx), df = R(-3) / (vx * vx * vx);
+        o.dX = o.dX * f + o.v * df * p.xs * sg;
+        o.dgD = o.v * df * (-vx * p.igd);
+        o.dgL = o.dgL * f;
+        o.v = o.v * f;
+        return o;
+    }
+    const R q = X * p.igd;
+    const R D = p.dopa * dexp<R>(-((q * q) * R(MARFA_LOG2_F)));
+    o.v = D;
+    o.dX = D * R(-2) * R(MARFA_LOG2_F) * q * p.igd;
+    o.dgD = D * (R(2) * R(MARFA_LOG2_F) * q * q - R(1)) * p.igd;
+    o.dgL = R(0);
+    if (VX > (double)MARFA_SQRT14_F) {  // asymptote 2: Doppler + Lorentz wing (1 + F(VX))
+        const int I = (int)(VX / 0.5 - (double)1.00001f);
+        const R F = R(2) * (R(ASYM_W[I - 1]) * (R(ASYM_U[I]) - vx) + R(ASYM_W[I]) * (vx - R(ASYM_U[I - 1])));
+        const R dF = R(2) * (R(ASYM_W[I]) - R(ASYM_W[I - 1]));
+        const R ix2 = R(1) / (X * X);
+        const R T2 = p.lorc * ix2 * (R(1) + F);
+        o.v += T2;
+        o.dX += R(-2) * T2 / X + p.lorc * ix2 * dF * p.xs * sg;
+        o.dgD += p.lorc * ix2 * dF * (-vx * p.igd);
+        o.dgL = p.S / R(MARFA_PI_F) * ix2 * (R(1) + F);
+    }
+    return o;
 }
 
 // ---------------------------------------------------------------- accumulators

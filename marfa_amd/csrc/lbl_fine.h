@@ -333,20 +333,10 @@ __device__ inline void store_level_node(double *g, int k, int t, int qloc, doubl
     }
 }
 
+// ---- stages of one (record, tile) work item, shared by the forward and adjoint kernels
+
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
-    __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
-    __shared__ double s_u[STAGE_CAP];     // staged centre lines
-    __shared__ int32_t s_j[STAGE_CAP];
-    __shared__ int32_t s_b[STAGE_NB + 1];
-    __shared__ int s_cnt[BLOCK / 64];
-    const int nitems = a.nlev * a.nrec * TILES;
-    const int w = xcd_item(nitems);
-    if (w >= nitems) return;
-    const int r = w / TILES, t = w % TILES;
-    const int lev = r / a.nrec, s = r % a.nrec;
-    const double sc = a.sscale[a.lev0 + lev];
-    FineCtx<R> f;
+__device__ inline void tile_ctx(const KArgs &a, int lev, int s, FineCtx<R> &f) {
     f.skey = a.skey + (int64_t)lev * a.nl;
     f.sidx = a.sidx + (int64_t)lev * a.nl;
     f.shp = reinterpret_cast<const LineShape<R> *>(a.shp) + (int64_t)lev * a.nl;
@@ -359,38 +349,40 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     f.wlo = a.lo[s];
     f.whi = a.hi[s];
     f.ctl = make_ctl(a, a.lev0 + lev);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const GlobalSrc<R> gsrc(f);
+}
 
-    // ---- stage the centre lines of the tile's fine reach: order-preserving compaction of
-    // the sorted candidates (one ballot per wave, wave counts through LDS)
+// Stage the centre lines of tile t's fine reach: order-preserving compaction of the sorted
+// candidates (one ballot per wave, wave counts through LDS), then the local bucket table.
+// Returns false (uniformly) when the reach holds more than STAGE_CAP centre lines.
+template <typename R>
+__device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32_t *s_j, int32_t *s_b,
+                                  int *s_cnt, CentreLds<R> &lsrc) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const double ustart = (double)t - STAGE_REACH, uend = (double)t + 1.0 + STAGE_REACH;
     int nst = 0;
-    {
-        int J0, J1;
-        cand(f, ustart, uend, J0, J1);
-        for (int base = J0; base < J1; base += BLOCK) {
-            const int j = base + tid;
-            double u = 0.0;
-            const bool keep = j < J1 && classify(f, j, u) == 2 && u >= ustart && u < uend;
-            const uint64_t m = __ballot(keep);
-            if (lane == 0) s_cnt[wv] = __popcll(m);
-            __syncthreads();
-            int off = nst, tot = 0;
-            for (int w2 = 0; w2 < BLOCK / 64; w2++) {
-                if (w2 < wv) off += s_cnt[w2];
-                tot += s_cnt[w2];
-            }
-            off += __popcll(m & ((1ull << lane) - 1ull));
-            if (keep && off < STAGE_CAP) {
-                s_u[off] = u;
-                s_j[off] = j;
-            }
-            nst += tot;
-            __syncthreads();
+    int J0, J1;
+    cand(f, ustart, uend, J0, J1);
+    for (int base = J0; base < J1; base += BLOCK) {
+        const int j = base + tid;
+        double u = 0.0;
+        const bool keep = j < J1 && classify(f, j, u) == 2 && u >= ustart && u < uend;
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) s_cnt[wv] = __popcll(m);
+        __syncthreads();
+        int off = nst, tot = 0;
+        for (int w2 = 0; w2 < BLOCK / 64; w2++) {
+            if (w2 < wv) off += s_cnt[w2];
+            tot += s_cnt[w2];
         }
+        off += __popcll(m & ((1ull << lane) - 1ull));
+        if (keep && off < STAGE_CAP) {
+            s_u[off] = u;
+            s_j[off] = j;
+        }
+        nst += tot;
+        __syncthreads();
     }
-    const bool staged = nst <= STAGE_CAP;  // uniform over the workgroup
+    const bool staged = nst <= STAGE_CAP;
     if (staged)
         for (int b = tid; b <= STAGE_NB; b += BLOCK) {
             const double x = ustart + (double)b * (1.0 / BUCKETS_PER_CM);
@@ -403,16 +395,23 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
             s_b[b] = lo;
         }
     __syncthreads();
-    CentreLds<R> lsrc;
     lsrc.su = s_u;
     lsrc.sj = s_j;
     lsrc.sb = s_b;
     lsrc.ustart = ustart;
     lsrc.shp = f.shp;
     lsrc.ctl = f.ctl;
+    return staged;
+}
 
-    // ---- grids 1..4: one wavefront per node, lines split over the lanes, fixed-order
-    // reduction (their reach spans several tiles: candidates come from global memory)
+// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grids 1..4: one
+// wavefront per node, lines split over the lanes, fixed-order reduction (their reach spans
+// several tiles: candidates from global memory); grids 5..9: one thread per node, centre
+// lines from the staged copy.  Ends with a barrier.
+template <typename R>
+__device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds<R> &lsrc, bool staged, int t,
+                                       double *g) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int nd = wv; nd < WAVE_NODES; nd += BLOCK / 64) {
         int k = 1;
         while (k < 4 && nd >= lvl_node_off(k + 1)) k++;
@@ -429,7 +428,6 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
         }
         if (lane == 0) store_level_node(g, k, t, qloc, vL, vP);
     }
-    // ---- grids 5..9: one thread per node, centre lines from LDS
     for (int nd = WAVE_NODES + tid; nd < NLVL_NODES; nd += BLOCK) {
         int k = 5;
         while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
@@ -444,30 +442,32 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
         store_level_node(g, k, t, qloc, aL.value(), aP.value());
     }
     __syncthreads();
+}
 
-    const double *cz = a.coarse + (int64_t)r * NCOARSE;
-    if (a.grids && r == 0) {
-        for (int e = tid; e < NACC; e += BLOCK) {
-            int k = 1;
-            while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
-            const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
-            a.grids[30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl] = g[e] / sc;
-        }
-        if (tid < 3) a.grids[tid * 10 + t] = cz[1 + 10 * tid + t] / sc;
-        if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
-    }
+// limiter of the cascade, src/Grids.f90:80-101: A1 is the new mid of the left child cell,
+// A2 of the right one; true when the limiter replaces the Lagrange value by the mean
+__device__ inline double cascade_a1(double P, double M, double L, bool &lim) {
+    double A = 0.375 * P + 0.75 * M - 0.125 * L;
+    lim = A < P || A < M;
+    return lim ? 0.5 * (P + M) : A;
+}
+__device__ inline double cascade_a2(double P, double M, double L, bool &lim) {
+    double A = 0.375 * L + 0.75 * M - 0.125 * P;
+    lim = A < L || A < M;
+    return lim ? 0.5 * (L + M) : A;
+}
 
-    // ---- cascade of this tile in FP64 (src/Grids.f90:80-139), grids 0..9
+// Cascade of tile t in FP64 from its grid-0 cell (cz: P, M, L of the 10 cells) down to
+// grid 9 (src/Grids.f90:104-139): afterwards g holds the cascaded values of every level.
+__device__ inline void tile_cascade(double *g, const double *cz, int t) {
+    const int tid = threadIdx.x;
     auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i) {
+        bool l1, l2;
         fP[i] += P;
-        double A = 0.375 * P + 0.75 * M - 0.125 * L;
-        if (A < P || A < M) A = 0.5 * (P + M);
-        fM[i] += A;
+        fM[i] += cascade_a1(P, M, L, l1);
         fL[i] += M;
         fP[i + 1] += M;
-        A = 0.375 * L + 0.75 * M - 0.125 * P;
-        if (A < L || A < M) A = 0.5 * (L + M);
-        fM[i + 1] += A;
+        fM[i + 1] += cascade_a2(P, M, L, l2);
         fL[i + 1] += L;
     };
     if (tid == 0) {
@@ -481,12 +481,47 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
             step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c);
         __syncthreads();
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
+    __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
+    __shared__ double s_u[STAGE_CAP];     // staged centre lines
+    __shared__ int32_t s_j[STAGE_CAP];
+    __shared__ int32_t s_b[STAGE_NB + 1];
+    __shared__ int s_cnt[BLOCK / 64];
+    const int nitems = a.nlev * a.nrec * TILES;
+    const int w = xcd_item(nitems);
+    if (w >= nitems) return;
+    const int r = w / TILES, t = w % TILES;
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const double sc = a.sscale[a.lev0 + lev];
+    const int tid = threadIdx.x;
+    FineCtx<R> f;
+    tile_ctx<R>(a, lev, s, f);
+    const GlobalSrc<R> gsrc(f);
+    CentreLds<R> lsrc;
+    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
+    tile_level_sums<R>(gsrc, lsrc, staged, t, g);
+
+    const double *cz = a.coarse + (int64_t)r * NCOARSE;
+    if (a.grids && r == 0) {
+        for (int e = tid; e < NACC; e += BLOCK) {
+            int k = 1;
+            while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
+            const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
+            a.grids[30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl] = g[e] / sc;
+        }
+        if (tid < 3) a.grids[tid * 10 + t] = cz[1 + 10 * tid + t] / sc;
+        if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
+    }
+    tile_cascade(g, cz, t);
 
     // ---- fine points: direct deposits summed in registers (thread owns the 4 points of
     // grid-9 cells tid and tid + 256), then the last cascade step and the record write
     const int cb9 = lvl_acc_off(9);
     const double mul = (double)a.outmul[a.lev0 + lev] / sc;
-    float *out = a.out + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
+    const int64_t ro = ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
     for (int h = 0; h < 2; h++) {
         const int c = tid + h * BLOCK;
         double v[4] = {0.0, 0.0, 0.0, 0.0};
@@ -507,16 +542,106 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
             if (a.grids && r == 0) a.grids[30690 + 2048 * t + 1 + il] = x / sc;
         }
         const double P = g[cb9 + c], M = g[cb9 + 512 + c], L = g[cb9 + 1024 + c];
-        double A = P * 0.375 + M * 0.75 - L * 0.125;
-        if (A < P || A < M) A = 0.5 * (P + M);
-        v[0] += A;
+        bool l1, l2;
+        v[0] += cascade_a1(P, M, L, l1);
         v[1] += M;
-        A = L * 0.375 + M * 0.75 - P * 0.125;
-        if (A < L || A < M) A = 0.5 * (L + M);
-        v[2] += A;
+        v[2] += cascade_a2(P, M, L, l2);
         v[3] += L;
 #pragma unroll
-        for (int e = 0; e < 4; e++) out[2048 * t + 1 + 4 * c + e] = (float)(v[e] * mul);
+        for (int e = 0; e < 4; e++) {
+            if (a.out64) a.out64[ro + 2048 * t + 1 + 4 * c + e] = v[e] * mul;
+            else a.out[ro + 2048 * t + 1 + 4 * c + e] = (float)(v[e] * mul);
+        }
     }
-    if (t == 0 && tid == 0) out[0] = (float)(cz[0] * mul);
+    if (t == 0 && tid == 0) {
+        if (a.out64) a.out64[ro] = cz[0] * mul;
+        else a.out[ro] = (float)(cz[0] * mul);
+    }
+}
+
+// ---- adjoint of the deposit + cascade of one (record, tile)
+// gout: dL/d(record point) in the records' layout; gbar: per record of the batch,
+// dL/d(pre-cascade node) in the reference's grid layout (grids 0..9 P|mid|L blocks, then
+// RK(1..20481)), for samples scaled by the level's power-of-two scale.  The tile recomputes
+// its grid 1..9 sums and the cascade (the limiter's choices need the cascaded values), then
+// runs the transposed cascade from its 2,048 points up to its grid-0 cell.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const float *gout, float *gbar) {
+    __shared__ double g[NACC];
+    __shared__ float gb[NACC];            // adjoints of the cascaded values, grids 1..9
+    __shared__ double s_u[STAGE_CAP];
+    __shared__ int32_t s_j[STAGE_CAP];
+    __shared__ int32_t s_b[STAGE_NB + 1];
+    __shared__ int s_cnt[BLOCK / 64];
+    const int nitems = a.nlev * a.nrec * TILES;
+    const int w = xcd_item(nitems);
+    if (w >= nitems) return;
+    const int r = w / TILES, t = w % TILES;
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const double sc = a.sscale[a.lev0 + lev];
+    const int tid = threadIdx.x;
+    FineCtx<R> f;
+    tile_ctx<R>(a, lev, s, f);
+    const GlobalSrc<R> gsrc(f);
+    CentreLds<R> lsrc;
+    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
+    tile_level_sums<R>(gsrc, lsrc, staged, t, g);
+    const double *cz = a.coarse + (int64_t)r * NCOARSE;
+    tile_cascade(g, cz, t);
+
+    const double mul = (double)a.outmul[a.lev0 + lev] / sc;
+    const float *go = gout + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
+    float *gr = gbar + (int64_t)r * (30690 + NT);
+    // fine points: the direct deposits' adjoint is the record's
+    for (int e = tid; e < 2048; e += BLOCK) gr[30690 + 2048 * t + 1 + e] = (float)(go[2048 * t + 1 + e] * mul);
+    if (t == 0 && tid == 0) gr[30690] = (float)(go[0] * mul);
+    // grid 9 from the last cascade step: points 4c+2..4c+5 of the tile get A1, M, A2, L
+    {
+        const int cb9 = lvl_acc_off(9);
+        for (int c = tid; c < 512; c += BLOCK) {
+            const double P = g[cb9 + c], M = g[cb9 + 512 + c], L = g[cb9 + 1024 + c];
+            bool l1, l2;
+            cascade_a1(P, M, L, l1);
+            cascade_a2(P, M, L, l2);
+            const int p0 = 2048 * t + 1 + 4 * c;
+            const double r0 = go[p0] * mul, r1 = go[p0 + 1] * mul, r2 = go[p0 + 2] * mul, r3 = go[p0 + 3] * mul;
+            gb[cb9 + c] = (float)(r0 * (l1 ? 0.5 : 0.375) + r2 * (l2 ? 0.0 : -0.125));
+            gb[cb9 + 512 + c] = (float)(r0 * (l1 ? 0.5 : 0.75) + r1 + r2 * (l2 ? 0.5 : 0.75));
+            gb[cb9 + 1024 + c] = (float)(r0 * (l1 ? 0.0 : -0.125) + r2 * (l2 ? 0.5 : 0.375) + r3);
+        }
+    }
+    __syncthreads();
+    // transposed steps k -> k+1 for k = 8..1, then grid 0
+    auto back = [&](double P, double M, double L, const float *cP, const float *cM, const float *cL, int i,
+                    float &oP, float &oM, float &oL) {
+        bool l1, l2;
+        cascade_a1(P, M, L, l1);
+        cascade_a2(P, M, L, l2);
+        const double m0 = cM[i], m1 = cM[i + 1];
+        oP = (float)(cP[i] + m0 * (l1 ? 0.5 : 0.375) + m1 * (l2 ? 0.0 : -0.125));
+        oM = (float)(m0 * (l1 ? 0.5 : 0.75) + cL[i] + cP[i + 1] + m1 * (l2 ? 0.5 : 0.75));
+        oL = (float)(m0 * (l1 ? 0.0 : -0.125) + m1 * (l2 ? 0.5 : 0.375) + cL[i + 1]);
+    };
+    for (int k = 8; k >= 1; k--) {
+        const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
+        for (int c = tid; c < nk; c += BLOCK)
+            back(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], gb + fb, gb + fb + 2 * nk, gb + fb + 4 * nk, 2 * c,
+                 gb[cb + c], gb[cb + nk + c], gb[cb + 2 * nk + c]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int k1 = lvl_acc_off(1);
+        float oP, oM, oL;
+        back(cz[1 + t], cz[11 + t], cz[21 + t], gb + k1, gb + k1 + 2, gb + k1 + 4, 0, oP, oM, oL);
+        gr[t] = oP;
+        gr[10 + t] = oM;
+        gr[20 + t] = oL;
+    }
+    // grids 1..9 of the tile into the reference layout
+    for (int e = tid; e < NACC; e += BLOCK) {
+        int k = 1;
+        while (k < 9 && e >= lvl_acc_off(k + 1)) k++;
+        const int nk = 1 << k, o = e - lvl_acc_off(k), role = o / nk, cl = o % nk;
+        gr[30 * ((1 << k) - 1) + role * (10 << k) + t * nk + cl] = gb[e];
+    }
 }

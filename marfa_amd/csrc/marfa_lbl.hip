@@ -76,6 +76,7 @@ struct KArgs {
     int shape_id, chi_id, mol;
     double *coarse;                           // [nlev*nrec][NCOARSE]
     float *out;                               // [lev][nrec][NT] (absolute level index)
+    double *out64;                            // same layout in FP64 (instead of out) or null
     double *grids;  // optional: pre-cascade sums of record 0 in the reference's layout
                     // (grids 0..9 as P|mid|L blocks, then RK(1..20481)), sample scale removed
 };
@@ -279,6 +280,7 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
 // ------------------------------------------------------------------ 4. fine + cascade
 constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes per tile
 #include "lbl_fine.h"
+#include "lbl_adjoint.h"
 
 // ------------------------------------------------------------------ sample counts
 __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
@@ -367,6 +369,7 @@ struct marfa_ctx {
         size_t bytes = 0;
     };
     Buf params[4], keys[2], idx[2], sorttmp, coarse, lvl, win, out, counts, shapes, buckets;
+    Buf gbar, gline, glev, gacc;  // adjoint
     hipEvent_t ev[5] = {};
     float timings[5] = {0, 0, 0, 0, 0};
     double *dbg_grids = nullptr;
@@ -440,8 +443,17 @@ struct LevelSet {  // host-side level state for one run
 };
 
 // Run levels x records.  Windows are level-independent; out is [nlev][nrec][NT].
+// Adjoint request of run(): gout = dL/d(record point) in the records' layout (device);
+// the per-line gradients accumulate in c->gacc ([nl][4] from line lmin), the per-level
+// T, P gradients in tp (host, [2][nlev]).
+struct AdjReq {
+    const float *gout = nullptr;
+    int64_t lmin = 0, nl = 0;
+    std::vector<double> *tp = nullptr;
+};
+
 int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
-        hipStream_t st, bool timed) {
+        hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr) {
     const int nlev = (int)L.P.size(), nrec = W.nrec;
     if (nrec == 0 || nlev == 0) return 0;
     for (float T : L.T)
@@ -458,7 +470,8 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     const size_t shp_bytes = job.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
 
     // level batches so that per-(line, level) arrays stay within a memory budget
-    const size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(NB + 1) * 4;
+    size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(NB + 1) * 4;
+    if (adj) per_level += (size_t)nrec * (30690 + NT) * 4 + (size_t)nl * 4 * 8;
     const size_t budget = (size_t)24 << 30;
     int lb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nlev, budget / std::max<size_t>(per_level, 1)));
 
@@ -492,6 +505,16 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, (uint64_t *)nullptr, (uint64_t *)nullptr,
                                        (int32_t *)nullptr, (int32_t *)nullptr, (int)nl, 0, 64, st);
     if (ensure(c, c->sorttmp, tmpb)) return MARFA_E_HIP;
+    const unsigned nblk = (unsigned)((nl + BLOCK - 1) / BLOCK);
+    if (adj) {
+        if (ensure(c, c->gbar, (size_t)lb * nrec * (30690 + NT) * 4) || ensure(c, c->gline, arr * 4 * 8) ||
+            ensure(c, c->glev, (size_t)lb * nblk * 2 * 8) || ensure(c, c->gacc, (size_t)nl * 4 * 8))
+            return MARFA_E_HIP;
+        HIPCHK(hipMemsetAsync(c->gacc.p, 0, (size_t)nl * 4 * 8, st));
+        adj->lmin = lmin;
+        adj->nl = nl;
+        adj->tp->assign(2 * (size_t)nlev, 0.0);
+    }
 
     // FP32 form: per-level power-of-two sample scale; FP64 form: 1
     std::vector<double> hsc(nlev, 1.0);
@@ -593,6 +616,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.mol = job.mol_code;
         ka.coarse = (double *)c->coarse.p;
         ka.out = out;
+        ka.out64 = out64;
         ka.grids = c->dbg_grids;
         ka.shp = c->shapes.p;
         ka.bst = (const int32_t *)c->buckets.p;
@@ -621,10 +645,50 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         }
         HIPCHK(hipGetLastError());
         if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
-        if (job.precision == MARFA_FP32K)
+        if (adj) {
+            // adjoint: dL/dnode of every record, then the line pass and its reductions
+            float *gbar = (float *)c->gbar.p;
+            if (job.precision == MARFA_FP32K)
+                hipLaunchKernelGGL(adjoint_tile_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+            else
+                hipLaunchKernelGGL(adjoint_tile_kernel<double>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+            HIPCHK(hipGetLastError());
+            AdjArgs ab;
+            ab.nu = c->d_nu;
+            ab.S = c->d_S;
+            ab.gair = c->d_gair;
+            ab.gself = c->d_gself;
+            ab.elow = c->d_elow;
+            ab.nT = c->d_nT;
+            ab.delta = c->d_delta;
+            ab.molIso = c->d_molIso;
+            ab.tips = c->d_tips;
+            ab.P = dP;
+            ab.pself = dps;
+            ab.gbar = gbar;
+            ab.gline = (double *)c->gline.p;
+            ab.glev = (double *)c->glev.p;
+            if (job.precision == MARFA_FP32K)
+                hipLaunchKernelGGL(adjoint_line_kernel<float>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+            else
+                hipLaunchKernelGGL(adjoint_line_kernel<double>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)c->gline.p, nb,
+                               nl, (double *)c->gacc.p);
+            HIPCHK(hipGetLastError());
+            std::vector<double> hl((size_t)nb * nblk * 2);
+            HIPCHK(hipMemcpyAsync(hl.data(), c->glev.p, hl.size() * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int q = 0; q < nb; q++)
+                for (unsigned b2 = 0; b2 < nblk; b2++) {
+                    (*adj->tp)[l0 + q] += hl[((size_t)q * nblk + b2) * 2];
+                    (*adj->tp)[nlev + l0 + q] += hl[((size_t)q * nblk + b2) * 2 + 1];
+                }
+        } else if (job.precision == MARFA_FP32K) {
             hipLaunchKernelGGL(fine_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka);
-        else
+        } else {
             hipLaunchKernelGGL(fine_kernel<double>, dim3(gfine), dim3(BLOCK), 0, st, ka);
+        }
         HIPCHK(hipGetLastError());
         if (timed) {
             HIPCHK(hipEventRecord(c->ev[4], st));
@@ -809,6 +873,59 @@ int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *o
         HIPCHK(hipStreamSynchronize(st));
     } else if (!async) {
         HIPCHK(hipStreamSynchronize(st));
+    }
+    return 0;
+}
+
+int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, void *stream) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (!out_dev) return set_err(c, MARFA_E_ARG, "null argument");
+    if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
+    (void)hipSetDevice(c->device);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    LevelSet L = level_set(c, *job);
+    rc = run(c, *job, L, w, nullptr, st, true, nullptr, out_dev);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int marfa_pt_tables_adjoint(marfa_ctx *c, const marfa_job *job, const float *grad_out, void *stream,
+                            double *grad_lines, double *grad_levels) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (!grad_out || !grad_lines || !grad_levels) return set_err(c, MARFA_E_ARG, "null argument");
+    if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
+    if (job->chi_id == MARFA_CHI_PERRIN && job->mol_code == 2)
+        return set_err(c, MARFA_E_ARG, "adjoint: the Perrin chi factor's temperature dependence is not differentiated");
+    (void)hipSetDevice(c->device);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    LevelSet L = level_set(c, *job);
+    const int nlev = (int)L.P.size();
+    std::fill(grad_lines, grad_lines + 4 * (size_t)c->n, 0.0);
+    std::fill(grad_levels, grad_levels + 2 * (size_t)nlev, 0.0);
+    if (w.nrec == 0) return 0;
+    std::vector<double> tp;
+    AdjReq req;
+    req.gout = grad_out;
+    req.tp = &tp;
+    rc = run(c, *job, L, w, nullptr, st, true, &req);
+    if (rc) return rc;
+    std::vector<double> acc((size_t)req.nl * 4);
+    HIPCHK(hipMemcpyAsync(acc.data(), c->gacc.p, acc.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t j = 0; j < req.nl; j++)
+        for (int q = 0; q < 4; q++) grad_lines[(size_t)q * c->n + req.lmin + j] = acc[(size_t)j * 4 + q];
+    // dL/dP_level and dL/dT_level; T also enters through pSelf = rho 10 k T / p0 (in the
+    // line pass) -- the density is held fixed
+    for (int l = 0; l < nlev; l++) {
+        grad_levels[l] = tp[l];
+        grad_levels[nlev + l] = tp[nlev + l];
     }
     return 0;
 }

@@ -63,6 +63,8 @@ def library() -> C.CDLL:
                                    C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
                                    C.c_int32, C.c_int32, fp]
     L.marfa_pt_tables.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, C.c_int32]
+    L.marfa_pt_tables_adjoint.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
+    L.marfa_pt_tables_f64.argtypes = [vp, C.POINTER(_Job), vp, vp]
     L.marfa_count_samples.argtypes = [vp, C.POINTER(_Job), i64p]
     L.marfa_grid_sums.argtypes = [vp, C.POINTER(_Job), C.c_int32, C.c_int32, dp, fp]
     L.marfa_last_timings.argtypes = [vp, fp]
@@ -73,8 +75,8 @@ def library() -> C.CDLL:
 # every symbol the header declares (tests check the library exports them)
 EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_version",
            "marfa_set_tips", "marfa_set_lines", "marfa_set_levels", "marfa_windows",
-           "marfa_lbl_scheme", "marfa_pt_tables", "marfa_count_samples", "marfa_grid_sums",
-           "marfa_last_timings")
+           "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint", "marfa_count_samples",
+           "marfa_grid_sums", "marfa_last_timings")
 
 
 def _p(a):
@@ -140,6 +142,7 @@ class LBLContext:
             arrs.append(np.ascontiguousarray(getattr(lines, f), np.float32))
         arrs.append(np.ascontiguousarray(lines.molIso, np.int32))
         self._check(self.lib.marfa_set_lines(self.h, len(arrs[0]), *[_p(a) for a in arrs]))
+        self.nlines = len(arrs[0])
 
     def set_levels(self, atm):
         P = np.ascontiguousarray(atm.P, np.float32)
@@ -196,6 +199,26 @@ class LBLContext:
         st = torch.cuda.current_stream().cuda_stream
         self.pt_tables_device(spec, out.data_ptr(), st, sync=True)
         return out
+
+    def pt_tables_f64_torch(self, spec: JobSpec):
+        """Records before their FP32 rounding, as an FP64 CUDA tensor."""
+        import torch
+        lo, _, _, _ = self.windows(spec)
+        out = torch.empty((self.nlev, lo.shape[0], NT), dtype=torch.float64, device="cuda")
+        j = spec._c()
+        self._check(self.lib.marfa_pt_tables_f64(self.h, C.byref(j), C.c_void_p(out.data_ptr()),
+                                                 C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return out
+
+    def pt_tables_adjoint(self, spec: JobSpec, grad_out_ptr: int, stream: int = 0):
+        """dL/d(S, gamma_air, n_T, delta) per line ([4][n], float64) and dL/d(T, P) per level
+        ([2][nlev]) from dL/d(records) in device memory (marfa_pt_tables' layout)."""
+        gl = np.zeros((4, self.nlines), np.float64)
+        gv = np.zeros((2, self.nlev), np.float64)
+        j = spec._c()
+        self._check(self.lib.marfa_pt_tables_adjoint(self.h, C.byref(j), C.c_void_p(grad_out_ptr),
+                                                     C.c_void_p(stream) if stream else None, _p(gl), _p(gv)))
+        return gl, gv
 
     def count_samples(self, spec: JobSpec, coarse: bool = False):
         n = np.zeros(2, np.int64)

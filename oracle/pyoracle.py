@@ -150,6 +150,17 @@ class OracleJob:
         assert rc == 0
         return out, cnt
 
+    def line_grids(self, lev, r, i, precision=F64):
+        """Pre-cascade grids (NGRID + NT) of record r at level lev holding only line i's
+        deposits: the nodes line i samples and its value at each."""
+        g = np.zeros(NGRID + NT, np.float64)
+        rk = np.zeros(NT, np.float64)
+        lib().mo_record(precision, C.byref(self.lines), C.byref(self.job), _ptr(self.tips),
+                        float(self.P[lev]), float(self.T[lev]), float(self.rho[lev]),
+                        float(self.origin[r]), int(i), int(i) + 1,
+                        rk.ctypes.data_as(C.POINTER(C.c_double)), _ptr(g), None)
+        return g
+
     def direct(self, lev, r, i0=0, stride=1):
         npts = len(range(i0, NT, stride))
         out = np.zeros(npts, np.float64)
