@@ -26,34 +26,6 @@ __device__ inline int bucket_of(double nus, double base, int NB) {
     return (int)x;
 }
 
-// bst[b] = first sorted line whose bucket is >= b, b = 0..NB (each entry written once).
-// Line j fills the buckets (bucket(j-1), bucket(j)]; gaps longer than a few buckets (empty
-// stretches of spectrum) are filled by the whole wavefront together.
-__global__ __launch_bounds__(BLOCK) void bucket_kernel(const uint64_t *skey, int64_t nl, double base, int NB,
-                                                       int32_t *bst) {
-    const int lev = blockIdx.y;
-    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const uint64_t *k = skey + (int64_t)lev * nl;
-    int32_t *o = bst + (int64_t)lev * (NB + 1);
-    int b0 = 0, b1 = -1;  // this lane's gap [b0, b1]
-    if (j <= nl) {
-        b0 = (j == 0 ? -1 : bucket_of(double_of(k[j - 1]), base, NB)) + 1;
-        b1 = j == nl ? NB : bucket_of(double_of(k[j]), base, NB);
-    }
-    constexpr int SHORT = 8;
-    if (b1 - b0 < SHORT)
-        for (int b = b0; b <= b1; b++) o[b] = (int32_t)j;
-    uint64_t pending = __ballot(b1 - b0 >= SHORT);
-    while (pending) {
-        const int l = __builtin_ctzll(pending);
-        pending &= pending - 1;
-        const int s = __shfl(b0, l), e = __shfl(b1, l);
-        const int32_t v = (int32_t)__shfl((int)j, l);
-        for (int b = s + lane; b <= e; b += 64) o[b] = v;
-    }
-}
-
 // profile parameters of every line, in sorted order, for the fine gathers
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void shape_kernel(const int32_t *sidx, int64_t nl, int64_t lmin,

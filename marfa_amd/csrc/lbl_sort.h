@@ -1,0 +1,144 @@
+// lbl_sort.h -- per-level ordering of the lines by shifted centre, and the bucket index.
+// Included by marfa_lbl.hip inside its kernel namespace.
+//
+// A counting sort on the 1/512 cm-1 buckets the fine gathers use: histogram (integer
+// atomics), exclusive scan (the bucket index bst itself), scatter, then an insertion sort
+// of each bucket's few lines by (shifted centre, line index).  The result is the stable
+// order by shifted centre whatever the scheduling, and every step is a plain kernel (no
+// library call, no memset), so the whole pipeline can be captured in a hipGraph.
+#pragma once
+
+constexpr int SCAN_CHUNK = BLOCK * 16;  // buckets per scan block
+
+__global__ __launch_bounds__(BLOCK) void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) p[i] = v;
+}
+
+// cnt[lev][b] += 1 for every line of the level (cnt zero on entry)
+__global__ __launch_bounds__(BLOCK) void bucket_hist_kernel(const uint64_t *keys, int64_t nl, double base, int NB,
+                                                            int32_t *cnt) {
+    const int lev = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nl) return;
+    const int b = bucket_of(double_of(keys[(int64_t)lev * nl + j]), base, NB);
+    atomicAdd(cnt + (int64_t)lev * NB + b, 1);
+}
+
+__device__ inline int block_exclusive_scan(int x, int *tmp, int &total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = x;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) tmp[wv] = inc;
+    __syncthreads();
+    int woff = 0;
+    total = 0;
+    for (int w = 0; w < BLOCK / 64; w++) {
+        if (w < wv) woff += tmp[w];
+        total += tmp[w];
+    }
+    __syncthreads();
+    return woff + inc - x;
+}
+
+// part[lev][blk] = sum of the block's SCAN_CHUNK counts
+__global__ __launch_bounds__(BLOCK) void scan_partial_kernel(const int32_t *cnt, int NB, int nblk, int32_t *part) {
+    __shared__ int tmp[BLOCK / 64];
+    const int lev = blockIdx.y, blk = blockIdx.x;
+    const int32_t *c = cnt + (int64_t)lev * NB;
+    int s = 0;
+    for (int e = 0; e < 16; e++) {
+        const int b = blk * SCAN_CHUNK + e * BLOCK + threadIdx.x;
+        if (b < NB) s += c[b];
+    }
+    int total;
+    block_exclusive_scan(s, tmp, total);
+    if (threadIdx.x == 0) part[(int64_t)lev * nblk + blk] = total;
+}
+
+// exclusive scan of part[lev][0..nblk) in place, one workgroup per level
+__global__ __launch_bounds__(BLOCK) void scan_top_kernel(int32_t *part, int nblk) {
+    __shared__ int tmp[BLOCK / 64];
+    int32_t *p = part + (int64_t)blockIdx.y * nblk;
+    int carry = 0;
+    for (int base = 0; base < nblk; base += BLOCK) {
+        const int i = base + threadIdx.x;
+        const int x = i < nblk ? p[i] : 0;
+        int total;
+        const int ex = block_exclusive_scan(x, tmp, total);
+        if (i < nblk) p[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// bst[lev][b] = first sorted position of bucket b (b = 0..NB; bst[NB] = nl)
+__global__ __launch_bounds__(BLOCK) void scan_final_kernel(const int32_t *cnt, int NB, int nblk, const int32_t *part,
+                                                           int32_t *bst) {
+    __shared__ int tmp[BLOCK / 64];
+    const int lev = blockIdx.y, blk = blockIdx.x;
+    const int32_t *c = cnt + (int64_t)lev * NB;
+    int32_t *o = bst + (int64_t)lev * (NB + 1);
+    // each thread owns 16 consecutive buckets of the chunk
+    const int b0 = blk * SCAN_CHUNK + threadIdx.x * 16;
+    int v[16], s = 0;
+    for (int e = 0; e < 16; e++) {
+        v[e] = b0 + e < NB ? c[b0 + e] : 0;
+        s += v[e];
+    }
+    int total;
+    int run = part[(int64_t)lev * nblk + blk] + block_exclusive_scan(s, tmp, total);
+    for (int e = 0; e < 16; e++) {
+        if (b0 + e < NB) o[b0 + e] = run;
+        run += v[e];
+    }
+    if (blk == nblk - 1 && threadIdx.x == BLOCK - 1) o[NB] = run;
+}
+
+// place every line in its bucket's range (order inside the bucket set by the atomics);
+// cnt is consumed (back to zero)
+__global__ __launch_bounds__(BLOCK) void bucket_scatter_kernel(const uint64_t *keys, const int32_t *idx, int64_t nl,
+                                                               double base, int NB, const int32_t *bst, int32_t *cnt,
+                                                               uint64_t *skey, int32_t *sidx) {
+    const int lev = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nl) return;
+    const int64_t o = (int64_t)lev * nl;
+    const uint64_t k = keys[o + j];
+    const int b = bucket_of(double_of(k), base, NB);
+    const int pos = bst[(int64_t)lev * (NB + 1) + b] + atomicSub(cnt + (int64_t)lev * NB + b, 1) - 1;
+    skey[o + pos] = k;
+    sidx[o + pos] = idx[o + j];
+}
+
+// insertion sort of each bucket by (key, line index): the stable order by shifted centre
+__global__ __launch_bounds__(BLOCK) void bucket_order_kernel(int NB, int64_t nl, const int32_t *bst, uint64_t *skey,
+                                                             int32_t *sidx) {
+    const int lev = blockIdx.y;
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= NB) return;
+    const int32_t *s = bst + (int64_t)lev * (NB + 1);
+    const int lo = s[b], hi = s[b + 1];
+    if (hi - lo < 2) return;
+    uint64_t *k = skey + (int64_t)lev * nl;
+    int32_t *x = sidx + (int64_t)lev * nl;
+    for (int i = lo + 1; i < hi; i++) {
+        const uint64_t kk = k[i];
+        const int32_t xx = x[i];
+        int p = i - 1;
+        while (p >= lo && (k[p] > kk || (k[p] == kk && x[p] > xx))) {
+            k[p + 1] = k[p];
+            x[p + 1] = x[p];
+            p--;
+        }
+        k[p + 1] = kk;
+        x[p + 1] = xx;
+    }
+}
+
+// FP32 form: sample scale 2^-emax of every level from the per-level maximum exponent
+__global__ void scale_kernel(const int *emax, int nlev, double *sscale) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nlev) sscale[l] = emax[l] > -2000 ? __builtin_ldexp(1.0, -emax[l]) : 1.0;
+}

@@ -3,8 +3,8 @@
 // Pipeline for a batch of atmospheric levels (one stream, no atomics in any sum):
 //   1. spectroscopy_kernel  one thread per (line, level): shifted centre, Doppler and
 //      Lorentz HWHM, S(T) in FP64 (src/Spectroscopy.f90:38-145, src/LBL.f90:64-73).
-//   2. per-level radix sort of the shifted centres (the gather needs lines ordered by
-//      their position inside each subinterval).
+//   2. per-level counting sort of the shifted centres on 1/512 cm-1 buckets (the gather
+//      needs lines ordered by position; the bucket starts are its index).
 //   3. coarse_kernel  one workgroup per record (level, 10 cm-1 subinterval): every line of
 //      the record's window is evaluated at the 21 positions of the 1 cm-1 grid and the
 //      samples are summed per (cell, edge) in registers, then reduced across the
@@ -17,7 +17,6 @@
 //      order.  The tile then runs the cascade (src/Grids.f90:80-139) in FP64 from its
 //      level-0 cell down to the 2,048 points of the record it owns.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -281,6 +280,7 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
 constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes per tile
 #include "lbl_fine.h"
 #include "lbl_adjoint.h"
+#include "lbl_sort.h"
 
 // ------------------------------------------------------------------ sample counts
 __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
@@ -353,6 +353,7 @@ struct marfa_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    size_t budget = (size_t)24 << 30;  // bytes of per-(line, level) scratch per level batch
     // inputs
     int64_t n = 0;
     std::vector<double> h_nu;
@@ -368,8 +369,14 @@ struct marfa_ctx {
         void *p = nullptr;
         size_t bytes = 0;
     };
-    Buf params[4], keys[2], idx[2], sorttmp, coarse, lvl, win, out, counts, shapes, buckets;
-    Buf gbar, gline, glev, gacc;  // adjoint
+    // per-run scratch; a batched job borrows its lane's set (s points there)
+    struct Scratch {
+        Buf params[4], keys[2], idx[2], coarse, shapes, buckets, hist, spart;
+        Buf gbar, gline, glev, gacc;  // adjoint
+    };
+    Scratch own;
+    Scratch *s = &own;
+    Buf lvl, win, out, counts;  // level and window arrays of the last run, records, counters
     hipEvent_t ev[5] = {};
     float timings[5] = {0, 0, 0, 0, 0};
     double *dbg_grids = nullptr;
@@ -398,6 +405,17 @@ int ensure(marfa_ctx *c, marfa_ctx::Buf &b, size_t bytes) {
     HIPCHK(hipMalloc(&b.p, want));
     b.bytes = want;
     return 0;
+}
+
+void free_scratch(marfa_ctx::Scratch &s) {
+    marfa_ctx::Buf *bufs[] = {&s.params[0], &s.params[1], &s.params[2], &s.params[3], &s.keys[0], &s.keys[1],
+                              &s.idx[0], &s.idx[1], &s.coarse, &s.shapes, &s.buckets, &s.hist, &s.spart,
+                              &s.gbar, &s.gline, &s.glev, &s.gacc};
+    for (auto *b : bufs) {
+        if (b->p) (void)hipFree(b->p);
+        b->p = nullptr;
+        b->bytes = 0;
+    }
 }
 
 struct Windows {
@@ -444,7 +462,7 @@ struct LevelSet {  // host-side level state for one run
 
 // Run levels x records.  Windows are level-independent; out is [nlev][nrec][NT].
 // Adjoint request of run(): gout = dL/d(record point) in the records' layout (device);
-// the per-line gradients accumulate in c->gacc ([nl][4] from line lmin), the per-level
+// the per-line gradients accumulate in c->s->gacc ([nl][4] from line lmin), the per-level
 // T, P gradients in tp (host, [2][nlev]).
 struct AdjReq {
     const float *gout = nullptr;
@@ -452,8 +470,12 @@ struct AdjReq {
     std::vector<double> *tp = nullptr;
 };
 
+// capture: the kernels only, for recording into a hipGraph -- the level and window arrays
+// must already be on the device (an earlier run of the same job on the same context) and
+// every buffer already large enough, so that no allocation, copy or synchronisation occurs.
 int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
-        hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr) {
+        hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr, bool capture = false) {
+    if (capture) timed = false;
     const int nlev = (int)L.P.size(), nrec = W.nrec;
     if (nrec == 0 || nlev == 0) return 0;
     for (float T : L.T)
@@ -470,62 +492,60 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     const size_t shp_bytes = job.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
 
     // level batches so that per-(line, level) arrays stay within a memory budget
-    size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(NB + 1) * 4;
+    size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(2 * NB + 1) * 4;
     if (adj) per_level += (size_t)nrec * (30690 + NT) * 4 + (size_t)nl * 4 * 8;
-    const size_t budget = (size_t)24 << 30;
+    const size_t budget = c->budget;
     int lb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nlev, budget / std::max<size_t>(per_level, 1)));
 
-    // level arrays [nlev]: P, T, outmul (float), pself, sscale (double)
-    if (ensure(c, c->lvl, (size_t)nlev * (3 * 4 + 2 * 8) + 64)) return MARFA_E_HIP;
+    // level arrays [nlev]: P, T, outmul (float), pself, sscale (double), max exponent (int)
+    if (!capture && ensure(c, c->lvl, (size_t)nlev * (4 * 4 + 2 * 8) + 64)) return MARFA_E_HIP;
     float *dP = (float *)c->lvl.p, *dT = dP + nlev, *dmul = dT + nlev;
     double *dps = (double *)(((uintptr_t)(dmul + nlev) + 15) & ~(uintptr_t)15), *dsc = dps + nlev;
-    HIPCHK(hipMemcpyAsync(dP, L.P.data(), nlev * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dT, L.T.data(), nlev * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dmul, L.outmul.data(), nlev * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
+    int *demax = (int *)(dsc + nlev);
+    if (!capture) {
+        HIPCHK(hipMemcpyAsync(dP, L.P.data(), nlev * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dT, L.T.data(), nlev * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dmul, L.outmul.data(), nlev * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
+    }
     // windows
-    if (ensure(c, c->win, (size_t)nrec * 24 + 64)) return MARFA_E_HIP;
+    if (!capture && ensure(c, c->win, (size_t)nrec * 24 + 64)) return MARFA_E_HIP;
     int64_t *dlo = (int64_t *)c->win.p, *dhi = dlo + nrec;
     double *dorg = (double *)(dhi + nrec);
-    HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dhi, W.hi.data(), nrec * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dorg, W.org.data(), nrec * 8, hipMemcpyHostToDevice, st));
+    if (!capture) {
+        HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dhi, W.hi.data(), nrec * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dorg, W.org.data(), nrec * 8, hipMemcpyHostToDevice, st));
+    }
 
     const size_t arr = (size_t)lb * nl;
     for (int q = 0; q < 4; q++)
-        if (ensure(c, c->params[q], arr * 8)) return MARFA_E_HIP;
+        if (ensure(c, c->s->params[q], arr * 8)) return MARFA_E_HIP;
     for (int q = 0; q < 2; q++) {
-        if (ensure(c, c->keys[q], arr * 8)) return MARFA_E_HIP;
-        if (ensure(c, c->idx[q], arr * 4)) return MARFA_E_HIP;
+        if (ensure(c, c->s->keys[q], arr * 8)) return MARFA_E_HIP;
+        if (ensure(c, c->s->idx[q], arr * 4)) return MARFA_E_HIP;
     }
-    if (ensure(c, c->coarse, (size_t)lb * nrec * NCOARSE * 8)) return MARFA_E_HIP;
-    if (ensure(c, c->shapes, arr * shp_bytes)) return MARFA_E_HIP;
-    if (ensure(c, c->buckets, (size_t)lb * (NB + 1) * 4)) return MARFA_E_HIP;
-    size_t tmpb = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                       (int32_t *)nullptr, (int32_t *)nullptr, (int)nl, 0, 64, st);
-    if (ensure(c, c->sorttmp, tmpb)) return MARFA_E_HIP;
+    if (ensure(c, c->s->coarse, (size_t)lb * nrec * NCOARSE * 8)) return MARFA_E_HIP;
+    if (ensure(c, c->s->shapes, arr * shp_bytes)) return MARFA_E_HIP;
+    if (ensure(c, c->s->buckets, (size_t)lb * (NB + 1) * 4)) return MARFA_E_HIP;
+    const int nsblk = (NB + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (ensure(c, c->s->hist, (size_t)lb * NB * 4) || ensure(c, c->s->spart, (size_t)lb * nsblk * 4))
+        return MARFA_E_HIP;
     const unsigned nblk = (unsigned)((nl + BLOCK - 1) / BLOCK);
     if (adj) {
-        if (ensure(c, c->gbar, (size_t)lb * nrec * (30690 + NT) * 4) || ensure(c, c->gline, arr * 4 * 8) ||
-            ensure(c, c->glev, (size_t)lb * nblk * 2 * 8) || ensure(c, c->gacc, (size_t)nl * 4 * 8))
+        if (ensure(c, c->s->gbar, (size_t)lb * nrec * (30690 + NT) * 4) || ensure(c, c->s->gline, arr * 4 * 8) ||
+            ensure(c, c->s->glev, (size_t)lb * nblk * 2 * 8) || ensure(c, c->s->gacc, (size_t)nl * 4 * 8))
             return MARFA_E_HIP;
-        HIPCHK(hipMemsetAsync(c->gacc.p, 0, (size_t)nl * 4 * 8, st));
+        HIPCHK(hipMemsetAsync(c->s->gacc.p, 0, (size_t)nl * 4 * 8, st));
         adj->lmin = lmin;
         adj->nl = nl;
         adj->tp->assign(2 * (size_t)nlev, 0.0);
     }
 
-    // FP32 form: per-level power-of-two sample scale; FP64 form: 1
-    std::vector<double> hsc(nlev, 1.0);
-    int *demax = nullptr;
-    if (job.precision == MARFA_FP32K) {
-        if (ensure(c, c->counts, (size_t)nlev * 4 + 64)) return MARFA_E_HIP;
-        demax = (int *)c->counts.p;
-        std::vector<int> init(nlev, -2000);
-        HIPCHK(hipMemcpyAsync(demax, init.data(), nlev * 4, hipMemcpyHostToDevice, st));
-    }
-    HIPCHK(hipMemcpyAsync(dsc, hsc.data(), nlev * 8, hipMemcpyHostToDevice, st));
+    // FP32 form: per-level power-of-two sample scale from the maximum exponent of S(T);
+    // FP64 form: 1 (the exponent stays at its initial -2000)
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(BLOCK), 0, st, demax, (int64_t)nlev, -2000);
+    HIPCHK(hipGetLastError());
 
     float tms[5] = {0, 0, 0, 0, 0};
     for (int l0 = 0; l0 < nlev; l0 += lb) {
@@ -547,12 +567,12 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         sa.T = dT + l0;
         sa.pself = dps + l0;
         sa.tips = c->d_tips;
-        sa.nus = (double *)c->params[0].p;
-        sa.gD = (double *)c->params[1].p;
-        sa.gL = (double *)c->params[2].p;
-        sa.ST = (double *)c->params[3].p;
-        sa.keys = (uint64_t *)c->keys[0].p;
-        sa.idx = (int32_t *)c->idx[0].p;
+        sa.nus = (double *)c->s->params[0].p;
+        sa.gD = (double *)c->s->params[1].p;
+        sa.gL = (double *)c->s->params[2].p;
+        sa.ST = (double *)c->s->params[3].p;
+        sa.keys = (uint64_t *)c->s->keys[0].p;
+        sa.idx = (int32_t *)c->s->idx[0].p;
         dim3 gs((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
         hipLaunchKernelGGL(spectroscopy_kernel, gs, dim3(BLOCK), 0, st, sa);
         HIPCHK(hipGetLastError());
@@ -560,35 +580,43 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             dim3 gm((unsigned)std::min<int64_t>((nl + BLOCK - 1) / BLOCK, 1024), (unsigned)nb);
             hipLaunchKernelGGL(smax_kernel, gm, dim3(BLOCK), 0, st, (const double *)sa.ST, nl, demax + l0);
             HIPCHK(hipGetLastError());
-            std::vector<int> em(nb);
-            HIPCHK(hipMemcpyAsync(em.data(), demax + l0, nb * 4, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (int q = 0; q < nb; q++) hsc[l0 + q] = em[q] > -2000 ? std::ldexp(1.0, -em[q]) : 1.0;
-            HIPCHK(hipMemcpyAsync(dsc + l0, hsc.data() + l0, nb * 8, hipMemcpyHostToDevice, st));
         }
+        hipLaunchKernelGGL(scale_kernel, dim3((nb + 63) / 64), dim3(64), 0, st, (const int *)demax + l0, nb, dsc + l0);
+        HIPCHK(hipGetLastError());
         if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
-        for (int q = 0; q < nb; q++) {
-            size_t tb = c->sorttmp.bytes;
-            hipError_t e = hipcub::DeviceRadixSort::SortPairs(
-                c->sorttmp.p, tb, (uint64_t *)c->keys[0].p + (size_t)q * nl,
-                (uint64_t *)c->keys[1].p + (size_t)q * nl, (int32_t *)c->idx[0].p + (size_t)q * nl,
-                (int32_t *)c->idx[1].p + (size_t)q * nl, (int)nl, 0, 64, st);
-            if (e != hipSuccess) return set_err(c, MARFA_E_HIP, std::string("radix sort: ") + hipGetErrorString(e));
+        {
+            // order by shifted centre: bucket counts, their scan (the bucket index), scatter,
+            // in-bucket order
+            int32_t *hist = (int32_t *)c->s->hist.p, *bst = (int32_t *)c->s->buckets.p;
+            int32_t *part = (int32_t *)c->s->spart.p;
+            const uint64_t *k0 = (const uint64_t *)c->s->keys[0].p;
+            const int32_t *i0 = (const int32_t *)c->s->idx[0].p;
+            const int64_t nh = (int64_t)nb * NB;
+            hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((nh + BLOCK - 1) / BLOCK, 65536)),
+                               dim3(BLOCK), 0, st, hist, nh, 0);
+            const dim3 gl((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
+            hipLaunchKernelGGL(bucket_hist_kernel, gl, dim3(BLOCK), 0, st, k0, nl, bbase, NB, hist);
+            hipLaunchKernelGGL(scan_partial_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB,
+                               nsblk, part);
+            hipLaunchKernelGGL(scan_top_kernel, dim3(1, nb), dim3(BLOCK), 0, st, part, nsblk);
+            hipLaunchKernelGGL(scan_final_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB,
+                               nsblk, (const int32_t *)part, bst);
+            hipLaunchKernelGGL(bucket_scatter_kernel, gl, dim3(BLOCK), 0, st, k0, i0, nl, bbase, NB,
+                               (const int32_t *)bst, hist, (uint64_t *)c->s->keys[1].p, (int32_t *)c->s->idx[1].p);
+            hipLaunchKernelGGL(bucket_order_kernel, dim3((NB + BLOCK - 1) / BLOCK, nb), dim3(BLOCK), 0, st, NB, nl,
+                               (const int32_t *)bst, (uint64_t *)c->s->keys[1].p, (int32_t *)c->s->idx[1].p);
+            HIPCHK(hipGetLastError());
         }
         {
             dim3 gsh((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
             if (job.precision == MARFA_FP32K)
-                hipLaunchKernelGGL(shape_kernel<float>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->idx[1].p, nl,
+                hipLaunchKernelGGL(shape_kernel<float>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->s->idx[1].p, nl,
                                    lmin, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
-                                   (const double *)dsc, l0, (LineShape<float> *)c->shapes.p);
+                                   (const double *)dsc, l0, (LineShape<float> *)c->s->shapes.p);
             else
-                hipLaunchKernelGGL(shape_kernel<double>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->idx[1].p, nl,
+                hipLaunchKernelGGL(shape_kernel<double>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->s->idx[1].p, nl,
                                    lmin, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
-                                   (const double *)dsc, l0, (LineShape<double> *)c->shapes.p);
-            HIPCHK(hipGetLastError());
-            dim3 gb((unsigned)((nl + 1 + BLOCK - 1) / BLOCK), (unsigned)nb);
-            hipLaunchKernelGGL(bucket_kernel, gb, dim3(BLOCK), 0, st, (const uint64_t *)c->keys[1].p, nl, bbase, NB,
-                               (int32_t *)c->buckets.p);
+                                   (const double *)dsc, l0, (LineShape<double> *)c->s->shapes.p);
             HIPCHK(hipGetLastError());
         }
         if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
@@ -602,8 +630,8 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.gD = sa.gD;
         ka.gL = sa.gL;
         ka.S = sa.ST;
-        ka.skey = (const uint64_t *)c->keys[1].p;
-        ka.sidx = (const int32_t *)c->idx[1].p;
+        ka.skey = (const uint64_t *)c->s->keys[1].p;
+        ka.sidx = (const int32_t *)c->s->idx[1].p;
         ka.lo = dlo;
         ka.hi = dhi;
         ka.org = dorg;
@@ -614,12 +642,12 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.shape_id = job.shape_id;
         ka.chi_id = job.chi_id;
         ka.mol = job.mol_code;
-        ka.coarse = (double *)c->coarse.p;
+        ka.coarse = (double *)c->s->coarse.p;
         ka.out = out;
         ka.out64 = out64;
         ka.grids = c->dbg_grids;
-        ka.shp = c->shapes.p;
-        ka.bst = (const int32_t *)c->buckets.p;
+        ka.shp = c->s->shapes.p;
+        ka.bst = (const int32_t *)c->s->buckets.p;
         ka.bbase = bbase;
         ka.NB = NB;
         const int nitems = nb * nrec;
@@ -647,7 +675,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
         if (adj) {
             // adjoint: dL/dnode of every record, then the line pass and its reductions
-            float *gbar = (float *)c->gbar.p;
+            float *gbar = (float *)c->s->gbar.p;
             if (job.precision == MARFA_FP32K)
                 hipLaunchKernelGGL(adjoint_tile_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
             else
@@ -666,18 +694,18 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             ab.P = dP;
             ab.pself = dps;
             ab.gbar = gbar;
-            ab.gline = (double *)c->gline.p;
-            ab.glev = (double *)c->glev.p;
+            ab.gline = (double *)c->s->gline.p;
+            ab.glev = (double *)c->s->glev.p;
             if (job.precision == MARFA_FP32K)
                 hipLaunchKernelGGL(adjoint_line_kernel<float>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
             else
                 hipLaunchKernelGGL(adjoint_line_kernel<double>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)c->gline.p, nb,
-                               nl, (double *)c->gacc.p);
+            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)c->s->gline.p, nb,
+                               nl, (double *)c->s->gacc.p);
             HIPCHK(hipGetLastError());
             std::vector<double> hl((size_t)nb * nblk * 2);
-            HIPCHK(hipMemcpyAsync(hl.data(), c->glev.p, hl.size() * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(hl.data(), c->s->glev.p, hl.size() * 8, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             for (int q = 0; q < nb; q++)
                 for (unsigned b2 = 0; b2 < nblk; b2++) {
@@ -740,8 +768,8 @@ int marfa_ctx_destroy(marfa_ctx *c) {
     void *ptrs[] = {c->d_nu, c->d_S, c->d_gair, c->d_gself, c->d_elow, c->d_nT, c->d_delta, c->d_molIso, c->d_tips};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    marfa_ctx::Buf *bufs[] = {&c->params[0], &c->params[1], &c->params[2], &c->params[3], &c->keys[0], &c->keys[1],
-                              &c->idx[0], &c->idx[1], &c->sorttmp, &c->coarse, &c->lvl, &c->win, &c->out, &c->counts, &c->shapes, &c->buckets};
+    free_scratch(c->own);
+    marfa_ctx::Buf *bufs[] = {&c->lvl, &c->win, &c->out, &c->counts};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
@@ -917,7 +945,7 @@ int marfa_pt_tables_adjoint(marfa_ctx *c, const marfa_job *job, const float *gra
     rc = run(c, *job, L, w, nullptr, st, true, &req);
     if (rc) return rc;
     std::vector<double> acc((size_t)req.nl * 4);
-    HIPCHK(hipMemcpyAsync(acc.data(), c->gacc.p, acc.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(acc.data(), c->s->gacc.p, acc.size() * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     for (int64_t j = 0; j < req.nl; j++)
         for (int q = 0; q < 4; q++) grad_lines[(size_t)q * c->n + req.lmin + j] = acc[(size_t)j * 4 + q];
@@ -995,8 +1023,8 @@ int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
         one.outmul = {1.f};
         if (ensure(c, c->lvl, 64 + 3 * 4 + 2 * 8)) return MARFA_E_HIP;
         for (int q = 0; q < 4; q++)
-            if (ensure(c, c->params[q], (size_t)nl * 8)) return MARFA_E_HIP;
-        if (ensure(c, c->keys[0], (size_t)nl * 8) || ensure(c, c->idx[0], (size_t)nl * 4)) return MARFA_E_HIP;
+            if (ensure(c, c->s->params[q], (size_t)nl * 8)) return MARFA_E_HIP;
+        if (ensure(c, c->s->keys[0], (size_t)nl * 8) || ensure(c, c->s->idx[0], (size_t)nl * 4)) return MARFA_E_HIP;
         if (ensure(c, c->win, (size_t)w.nrec * 24 + 64)) return MARFA_E_HIP;
         float *dP = (float *)c->lvl.p, *dT = dP + 1;
         double *dps = (double *)(((uintptr_t)(dP + 3) + 15) & ~(uintptr_t)15);
@@ -1024,12 +1052,12 @@ int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
         sa.T = dT;
         sa.pself = dps;
         sa.tips = c->d_tips;
-        sa.nus = (double *)c->params[0].p;
-        sa.gD = (double *)c->params[1].p;
-        sa.gL = (double *)c->params[2].p;
-        sa.ST = (double *)c->params[3].p;
-        sa.keys = (uint64_t *)c->keys[0].p;
-        sa.idx = (int32_t *)c->idx[0].p;
+        sa.nus = (double *)c->s->params[0].p;
+        sa.gD = (double *)c->s->params[1].p;
+        sa.gL = (double *)c->s->params[2].p;
+        sa.ST = (double *)c->s->params[3].p;
+        sa.keys = (uint64_t *)c->s->keys[0].p;
+        sa.idx = (int32_t *)c->s->idx[0].p;
         hipLaunchKernelGGL(spectroscopy_kernel, dim3((unsigned)((nl + BLOCK - 1) / BLOCK), 1), dim3(BLOCK), 0, st, sa);
         HIPCHK(hipGetLastError());
         KArgs ka = {};
