@@ -146,6 +146,55 @@ def adjoint_bench(steps=3):
                     "kernels); *_kernels_ms: HIP-event time of the library's kernels alone"}
 
 
+def batch_bench(steps=2, lanes=4, check_jobs=24):
+    """256 independent jobs (line counts 10^U(3,6), 10-100 levels, 20-500 cm-1 windows anywhere
+    in 10-20,000 cm-1, molar masses {18,44,16,28,48}, cutoffs {25,125,250}), FP32K, recorded
+    into one hipGraph after the autotuner's choice per job; aggregate points/s of a replay and
+    the per-job error spread against the FP64 oracle on one sampled record of `check_jobs`
+    jobs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from marfa_amd import synth, lbl
+    t0 = time.time()
+    jobs = synth.config3_jobs()
+    tips = synth.load_tips()
+    b = lbl.Batch(0, lanes=lanes)
+    b.set_tips(tips)
+    ids = [b.add_job(j) for j in jobs]
+    t1 = time.time()
+    variants, _ = b.tune()
+    t2 = time.time()
+    b.capture()
+    b.launch()  # warm replay
+    ms = [b.launch() for _ in range(steps)]
+    points = sum(len(j.atm) * b.shapes[i][1] * lbl.NT for i, j in zip(ids, jobs))
+    dt = float(np.mean(ms)) / 1e3
+    rng = np.random.default_rng(7)
+    errs = []
+    for i in rng.choice(len(jobs), size=min(check_jobs, len(jobs)), replace=False):
+        job = jobs[i]
+        nlev, nrec, _ = b.shapes[i]
+        if nrec == 0:
+            continue
+        lev, r = int(rng.integers(nlev)), int(rng.integers(nrec))
+        rec = b.records(int(i))[lev, r]
+        ref, _ = po.OracleJob(job, tips).record(po.F64, lev, r)
+        m = np.abs(ref) > np.finfo(np.float32).tiny
+        if m.any():
+            errs.append(float(np.max(np.abs(rec[m] - ref[m]) / np.abs(ref[m]))))
+    b.close()
+    e = np.array(errs)
+    return {"workload": "256 jobs: 10^U(3,6) lines, U{10..100} levels (P 90->1e-4 atm, T 735->170 K), "
+                        "20-500 cm-1 windows in 10-20000 cm-1, masses {18,44,16,28,48}, cutoffs {25,125,250}, FP32K",
+            "jobs": len(jobs), "lines": int(sum(len(j.lines) for j in jobs)), "points": int(points),
+            "graph_ms": float(np.mean(ms)), "pts_per_s": points / dt, "lanes": lanes,
+            "variants_chosen": {f"cap{lbl.VARIANTS[v][0]}{'_xcd' if lbl.VARIANTS[v][1] else ''}": int((variants == v).sum())
+                                for v in range(lbl.NVARIANTS)},
+            "setup_s": t1 - t0, "tune_s": t2 - t1,
+            "error_spread_vs_fp64": {"jobs_checked": len(errs), "min": float(e.min()), "median": float(np.median(e)),
+                                     "max": float(e.max())} if errs else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +205,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 drop-in timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-job batched graph timing")
     ap.add_argument("--cpu-samples", type=float, default=1.0e9,
                     help="profile samples in the reference CPU sample (~17 s at 5.8e7 samples/s)")
     ap.add_argument("--nlines", type=int, default=2_000_000)
@@ -271,6 +321,10 @@ def main():
         log(f"[bench] FP64 drop-in: {d64 * 1e3:.1f} ms/step")
         ctx.pt_tables_device(spec, out.data_ptr(), stream)  # restore FP32 records for accuracy
         torch.cuda.synchronize()
+
+    if not args.no_batch:
+        result["batched"] = batch_bench()
+        log(f"[bench] batched {result['batched']}")
 
     if not args.no_adjoint:
         result["adjoint"] = adjoint_bench()

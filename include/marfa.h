@@ -134,6 +134,35 @@ int marfa_pt_tables_f64(marfa_ctx *ctx, const marfa_job *job, double *out_dev, v
 int marfa_pt_tables_adjoint(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
                             double *grad_lines, double *grad_levels);
 
+/* ---- batches of independent jobs in one hipGraph (no counterpart in the reference, which
+ * runs one job per process) ----
+ * Each job has its own line list, levels, windows and records.  marfa_batch_tune assigns
+ * the jobs to `lanes` parallel branches (largest first onto the least loaded lane), times
+ * the fine-kernel variants (centre lines staged in LDS per tile: 512/1024/2048, XCD-aware
+ * record order on/off) on each job's first level and keeps the fastest, then runs every
+ * job once; marfa_batch_capture records all jobs into one hipGraph (fork/join over the
+ * lanes' streams); marfa_batch_launch replays it with one hipGraphLaunch.  Jobs of a lane
+ * share that lane's scratch (lane_budget_gb of per-level arrays, 0 = 8 GB). */
+typedef struct marfa_batch marfa_batch;
+int marfa_batch_create(int device, int32_t lanes, double lane_budget_gb, marfa_batch **out);
+int marfa_batch_destroy(marfa_batch *b);
+const char *marfa_batch_last_error(const marfa_batch *b);
+int marfa_batch_set_tips(marfa_batch *b, int32_t niso, int32_t ntemp, const float *tips);
+int marfa_batch_add_job(marfa_batch *b, const marfa_job *job, int64_t n, const double *nu, const float *S,
+                        const float *gair, const float *gself, const float *elow, const float *nT,
+                        const float *delta, const int32_t *molIso, int32_t nlev, const float *P,
+                        const float *T, const float *rho, int32_t *job_id);
+/* variants (optional, [jobs]): chosen variant per job; tune_ms (optional, [jobs][6]): the
+ * fine kernel's time for every variant on the job's first level */
+int marfa_batch_tune(marfa_batch *b, int32_t *variants, float *tune_ms);
+int marfa_batch_capture(marfa_batch *b);
+/* stream NULL: lane 0's stream.  sync != 0: wait and record the launch's duration */
+int marfa_batch_launch(marfa_batch *b, void *stream, int32_t sync);
+/* job < 0: only last_ms (duration of the last synchronous launch) */
+int marfa_batch_info(marfa_batch *b, int32_t job, int32_t *nlev, int32_t *nrec, float **records_dev,
+                     float *last_ms);
+int marfa_batch_records(marfa_batch *b, int32_t job, float *out_host);
+
 /* Work counts of the job, summed over levels and subintervals (computed on the device):
  * samples[0] = exact count of distinct profile evaluations (samples) of the reference's
  * grid geometry (21 + 2m per wing line, the walks and core points of a centre line);

@@ -100,7 +100,7 @@ struct GlobalSrc {
 // the staged reach [t - STAGE_REACH, t + 1 + STAGE_REACH) holds every line they need.
 constexpr double STAGE_REACH = 0.1328125;  // 4 H_5 + 4/512, exact in binary
 constexpr int STAGE_NB = 648;              // (1 + 2 STAGE_REACH) * 512 buckets
-constexpr int STAGE_CAP = 1024;            // staged lines; denser tiles read global memory
+constexpr int STAGE_CAP = 1024;            // default staged lines; denser tiles read global memory
 
 template <typename R>
 struct CentreLds {
@@ -325,10 +325,10 @@ __device__ inline void tile_ctx(const KArgs &a, int lev, int s, FineCtx<R> &f) {
 
 // Stage the centre lines of tile t's fine reach: order-preserving compaction of the sorted
 // candidates (one ballot per wave, wave counts through LDS), then the local bucket table.
-// Returns false (uniformly) when the reach holds more than STAGE_CAP centre lines.
+// Returns false (uniformly) when the reach holds more than cap centre lines.
 template <typename R>
 __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32_t *s_j, int32_t *s_b,
-                                  int *s_cnt, CentreLds<R> &lsrc) {
+                                  int *s_cnt, CentreLds<R> &lsrc, int cap = STAGE_CAP) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const double ustart = (double)t - STAGE_REACH, uend = (double)t + 1.0 + STAGE_REACH;
     int nst = 0;
@@ -347,14 +347,14 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
             tot += s_cnt[w2];
         }
         off += __popcll(m & ((1ull << lane) - 1ull));
-        if (keep && off < STAGE_CAP) {
+        if (keep && off < cap) {
             s_u[off] = u;
             s_j[off] = j;
         }
         nst += tot;
         __syncthreads();
     }
-    const bool staged = nst <= STAGE_CAP;
+    const bool staged = nst <= cap;
     if (staged)
         for (int b = tid; b <= STAGE_NB; b += BLOCK) {
             const double x = ustart + (double)b * (1.0 / BUCKETS_PER_CM);
@@ -455,15 +455,17 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t) {
     }
 }
 
-template <typename R>
+// CAP: centre lines staged in LDS per tile (LDS 27.5 KB + 12 B x CAP: occupancy against
+// staging); XCD: contiguous record ranges per XCD (xcd_item) or plain dispatch order.
+template <typename R, int CAP = STAGE_CAP, bool XCD = true>
 __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
-    __shared__ double s_u[STAGE_CAP];     // staged centre lines
-    __shared__ int32_t s_j[STAGE_CAP];
+    __shared__ double s_u[CAP];           // staged centre lines
+    __shared__ int32_t s_j[CAP];
     __shared__ int32_t s_b[STAGE_NB + 1];
     __shared__ int s_cnt[BLOCK / 64];
     const int nitems = a.nlev * a.nrec * TILES;
-    const int w = xcd_item(nitems);
+    const int w = XCD ? xcd_item(nitems) : (int)blockIdx.x;
     if (w >= nitems) return;
     const int r = w / TILES, t = w % TILES;
     const int lev = r / a.nrec, s = r % a.nrec;
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     tile_ctx<R>(a, lev, s, f);
     const GlobalSrc<R> gsrc(f);
     CentreLds<R> lsrc;
-    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
+    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc, CAP);
     tile_level_sums<R>(gsrc, lsrc, staged, t, g);
 
     const double *cz = a.coarse + (int64_t)r * NCOARSE;

@@ -282,6 +282,23 @@ constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes per
 #include "lbl_adjoint.h"
 #include "lbl_sort.h"
 
+// Fine-kernel variants (the batch autotuner's choices): centre lines staged in LDS per tile
+// {512, 1024, 2048} x contiguous record ranges per XCD {on, off}.  Variant 1 is the default.
+constexpr int NVARIANTS = 6;
+constexpr int VARIANT_CAP[NVARIANTS] = {512, 1024, 2048, 512, 1024, 2048};
+
+template <typename R>
+void launch_fine(int variant, dim3 g, hipStream_t st, const KArgs &ka) {
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((fine_kernel<R, 512, true>), g, dim3(BLOCK), 0, st, ka); break;
+    case 2: hipLaunchKernelGGL((fine_kernel<R, 2048, true>), g, dim3(BLOCK), 0, st, ka); break;
+    case 3: hipLaunchKernelGGL((fine_kernel<R, 512, false>), g, dim3(BLOCK), 0, st, ka); break;
+    case 4: hipLaunchKernelGGL((fine_kernel<R, 1024, false>), g, dim3(BLOCK), 0, st, ka); break;
+    case 5: hipLaunchKernelGGL((fine_kernel<R, 2048, false>), g, dim3(BLOCK), 0, st, ka); break;
+    default: hipLaunchKernelGGL((fine_kernel<R, 1024, true>), g, dim3(BLOCK), 0, st, ka); break;
+    }
+}
+
 // ------------------------------------------------------------------ sample counts
 __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
     // counts[2r] = distinct samples of the reference's geometry in record r,
@@ -354,6 +371,7 @@ struct marfa_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     size_t budget = (size_t)24 << 30;  // bytes of per-(line, level) scratch per level batch
+    int variant = 1;                   // fine-kernel variant (launch_fine)
     // inputs
     int64_t n = 0;
     std::vector<double> h_nu;
@@ -713,9 +731,9 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
                     (*adj->tp)[nlev + l0 + q] += hl[((size_t)q * nblk + b2) * 2 + 1];
                 }
         } else if (job.precision == MARFA_FP32K) {
-            hipLaunchKernelGGL(fine_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka);
+            launch_fine<float>(c->variant, dim3(gfine), st, ka);
         } else {
-            hipLaunchKernelGGL(fine_kernel<double>, dim3(gfine), dim3(BLOCK), 0, st, ka);
+            launch_fine<double>(c->variant, dim3(gfine), st, ka);
         }
         HIPCHK(hipGetLastError());
         if (timed) {
@@ -774,7 +792,7 @@ int marfa_ctx_destroy(marfa_ctx *c) {
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
@@ -1123,6 +1141,293 @@ int marfa_grid_sums(marfa_ctx *c, const marfa_job *job, int32_t level, int32_t r
 int marfa_last_timings(const marfa_ctx *c, float *ms5) {
     if (!c || !ms5) return MARFA_E_ARG;
     memcpy(ms5, c->timings, sizeof c->timings);
+    return 0;
+}
+
+}  // extern "C"
+
+// ====================================================================== batches of jobs
+// Many independent jobs (each its own line list, levels, windows and records) recorded into
+// one hipGraph: the jobs are spread over `lanes` parallel branches (fork/join on events);
+// the jobs of a lane run in order and share that lane's scratch.  A job is a context whose
+// scratch pointer, stream and TIPS are borrowed from the batch.
+struct marfa_batch {
+    int device = 0;
+    std::string err;
+    std::vector<marfa_ctx *> lanes;
+    struct Job {
+        marfa_ctx *ctx = nullptr;
+        marfa_job spec{};
+        LevelSet L;
+        Windows W;
+        int lane = 0;
+        double work = 0;  // lines x levels x records, for the lane assignment
+        bool ready = false;  // run once eagerly on its lane (buffers at their final size)
+        float tune_ms[NVARIANTS] = {};
+    };
+    std::vector<Job> jobs;
+    float *d_tips = nullptr;
+    size_t lane_budget = (size_t)8 << 30;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> join;
+    float last_ms = 0;
+};
+
+namespace {
+int bset_err(marfa_batch *b, int code, const std::string &m) {
+    if (b) b->err = m;
+    return code;
+}
+#define BHIPCHK(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return bset_err(b, MARFA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void release_job_ctx(marfa_ctx *c) {
+    // borrowed members are the batch's
+    c->s = &c->own;
+    c->d_tips = nullptr;
+    c->stream = nullptr;
+    marfa_ctx_destroy(c);
+}
+}  // namespace
+
+extern "C" {
+
+int marfa_batch_create(int device, int32_t lanes, double lane_budget_gb, marfa_batch **out) {
+    if (!out || lanes < 1 || lanes > 16) return MARFA_E_ARG;
+    *out = nullptr;
+    marfa_batch *b = new marfa_batch();
+    b->device = device;
+    if (lane_budget_gb > 0) b->lane_budget = (size_t)(lane_budget_gb * (double)(1u << 30));
+    for (int k = 0; k < lanes; k++) {
+        marfa_ctx *c = nullptr;
+        int rc = marfa_ctx_create(device, &c);
+        if (rc) {
+            for (auto *l : b->lanes) marfa_ctx_destroy(l);
+            delete b;
+            return rc;
+        }
+        b->lanes.push_back(c);
+    }
+    (void)hipSetDevice(device);
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&b->t0) != hipSuccess ||
+        hipEventCreate(&b->t1) != hipSuccess) {
+        for (auto *l : b->lanes) marfa_ctx_destroy(l);
+        delete b;
+        return MARFA_E_HIP;
+    }
+    b->join.resize(lanes);
+    for (auto &e : b->join) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    *out = b;
+    return 0;
+}
+
+int marfa_batch_destroy(marfa_batch *b) {
+    if (!b) return MARFA_E_ARG;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto &j : b->jobs) release_job_ctx(j.ctx);
+    for (auto *l : b->lanes) marfa_ctx_destroy(l);
+    if (b->d_tips) (void)hipFree(b->d_tips);
+    for (auto e : {b->fork, b->t0, b->t1})
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    delete b;
+    return 0;
+}
+
+const char *marfa_batch_last_error(const marfa_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int marfa_batch_set_tips(marfa_batch *b, int32_t niso, int32_t ntemp, const float *tips) {
+    if (!b || niso != 74 || ntemp != 501 || !tips) return bset_err(b, MARFA_E_ARG, "TIPS must be 74 x 501");
+    (void)hipSetDevice(b->device);
+    if (!b->d_tips) BHIPCHK(hipMalloc(&b->d_tips, (size_t)niso * ntemp * 4));
+    BHIPCHK(hipMemcpy(b->d_tips, tips, (size_t)niso * ntemp * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int marfa_batch_add_job(marfa_batch *b, const marfa_job *job, int64_t n, const double *nu, const float *S,
+                        const float *gair, const float *gself, const float *elow, const float *nT,
+                        const float *delta, const int32_t *molIso, int32_t nlev, const float *P, const float *T,
+                        const float *rho, int32_t *job_id) {
+    if (!b || !job || !job_id) return bset_err(b, MARFA_E_ARG, "null argument");
+    if (!b->d_tips) return bset_err(b, MARFA_E_STATE, "TIPS must be set first");
+    if (b->exec) return bset_err(b, MARFA_E_STATE, "the batch is already captured");
+    (void)hipSetDevice(b->device);
+    marfa_ctx *c = new marfa_ctx();
+    c->device = b->device;
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    auto fail = [&](int rc) {
+        b->err = c->err;
+        release_job_ctx(c);
+        return rc;
+    };
+    c->d_tips = b->d_tips;
+    c->tips_set = true;
+    c->budget = b->lane_budget;
+    int rc = marfa_set_lines(c, n, nu, S, gair, gself, elow, nT, delta, molIso);
+    if (!rc) rc = marfa_set_levels(c, nlev, P, T, rho);
+    if (!rc) rc = check_job(c, job);
+    if (rc) return fail(rc);
+    marfa_batch::Job J;
+    J.ctx = c;
+    J.spec = *job;
+    compute_windows(c->h_nu, *job, J.W);
+    J.L = level_set(c, *job);
+    const size_t total = (size_t)nlev * J.W.nrec * NT;
+    if (ensure(c, c->out, std::max<size_t>(total, 1) * 4)) return fail(MARFA_E_HIP);
+    int64_t lines = 0;
+    if (J.W.nrec > 0)
+        lines = *std::max_element(J.W.hi.begin(), J.W.hi.end()) - *std::min_element(J.W.lo.begin(), J.W.lo.end());
+    J.work = (double)std::max<int64_t>(lines, 1) * nlev * std::max(J.W.nrec, 1);
+    *job_id = (int32_t)b->jobs.size();
+    b->jobs.push_back(std::move(J));
+    return 0;
+}
+
+// Lane assignment (largest job first onto the least loaded lane), then per job: time every
+// fine-kernel variant on the job's first level and keep the fastest, then one full eager run
+// (records, and every buffer at its final size so that the capture allocates nothing).
+int marfa_batch_tune(marfa_batch *b, int32_t *variants, float *tune_ms) {
+    if (!b) return MARFA_E_ARG;
+    if (b->exec) return bset_err(b, MARFA_E_STATE, "the batch is already captured");
+    (void)hipSetDevice(b->device);
+    const int K = (int)b->lanes.size();
+    std::vector<int> order(b->jobs.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->jobs[x].work > b->jobs[y].work; });
+    std::vector<double> load(K, 0.0);
+    for (int j : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->jobs[j].lane = k;
+        load[k] += b->jobs[j].work;
+    }
+    for (size_t j = 0; j < b->jobs.size(); j++) {
+        auto &J = b->jobs[j];
+        marfa_ctx *c = J.ctx;
+        marfa_ctx *lane = b->lanes[J.lane];
+        c->s = &lane->own;
+        c->stream = lane->stream;
+        float *out = (float *)c->out.p;
+        int best = 1;
+        if (J.W.nrec > 0) {
+            LevelSet L1;
+            L1.P = {J.L.P[0]};
+            L1.T = {J.L.T[0]};
+            L1.pself = {J.L.pself[0]};
+            L1.outmul = {J.L.outmul[0]};
+            float bms = 0;
+            for (int v = 0; v < NVARIANTS; v++) {
+                c->variant = v;
+                int rc = run(c, J.spec, L1, J.W, out, lane->stream, true);
+                if (!rc) rc = run(c, J.spec, L1, J.W, out, lane->stream, true);  // second run timed warm
+                if (rc) {
+                    b->err = c->err;
+                    return rc;
+                }
+                J.tune_ms[v] = c->timings[3];
+                if (v == 0 || J.tune_ms[v] < bms) {
+                    bms = J.tune_ms[v];
+                    best = v;
+                }
+            }
+        }
+        c->variant = best;
+        int rc = run(c, J.spec, J.L, J.W, out, lane->stream, false);
+        if (rc) {
+            b->err = c->err;
+            return rc;
+        }
+        J.ready = true;
+        if (variants) variants[j] = best;
+        if (tune_ms)
+            for (int v = 0; v < NVARIANTS; v++) tune_ms[j * NVARIANTS + v] = J.tune_ms[v];
+    }
+    for (auto *l : b->lanes) BHIPCHK(hipStreamSynchronize(l->stream));
+    return 0;
+}
+
+int marfa_batch_capture(marfa_batch *b) {
+    if (!b) return MARFA_E_ARG;
+    if (b->exec) return 0;
+    for (const auto &J : b->jobs)
+        if (!J.ready) return bset_err(b, MARFA_E_STATE, "tune the batch before capturing it");
+    (void)hipSetDevice(b->device);
+    for (auto *l : b->lanes) BHIPCHK(hipStreamSynchronize(l->stream));
+    hipStream_t s0 = b->lanes[0]->stream;
+    BHIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+    int rc = 0;
+    if (hipEventRecord(b->fork, s0) != hipSuccess) rc = MARFA_E_HIP;
+    for (size_t k = 1; !rc && k < b->lanes.size(); k++)
+        if (hipStreamWaitEvent(b->lanes[k]->stream, b->fork, 0) != hipSuccess) rc = MARFA_E_HIP;
+    for (auto &J : b->jobs) {
+        if (rc) break;
+        marfa_ctx *lane = b->lanes[J.lane];
+        J.ctx->s = &lane->own;
+        J.ctx->stream = lane->stream;
+        rc = run(J.ctx, J.spec, J.L, J.W, (float *)J.ctx->out.p, lane->stream, false, nullptr, nullptr, true);
+        if (rc) b->err = J.ctx->err;
+    }
+    for (size_t k = 1; !rc && k < b->lanes.size(); k++) {
+        if (hipEventRecord(b->join[k], b->lanes[k]->stream) != hipSuccess ||
+            hipStreamWaitEvent(s0, b->join[k], 0) != hipSuccess)
+            rc = MARFA_E_HIP;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s0, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) b->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return rc ? rc : MARFA_E_HIP;
+    }
+    b->graph = g;
+    BHIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+int marfa_batch_launch(marfa_batch *b, void *stream, int32_t sync) {
+    if (!b) return MARFA_E_ARG;
+    if (!b->exec) return bset_err(b, MARFA_E_STATE, "capture the batch first");
+    (void)hipSetDevice(b->device);
+    hipStream_t st = stream ? (hipStream_t)stream : b->lanes[0]->stream;
+    BHIPCHK(hipEventRecord(b->t0, st));
+    BHIPCHK(hipGraphLaunch(b->exec, st));
+    BHIPCHK(hipEventRecord(b->t1, st));
+    if (sync) {
+        BHIPCHK(hipEventSynchronize(b->t1));
+        BHIPCHK(hipEventElapsedTime(&b->last_ms, b->t0, b->t1));
+    }
+    return 0;
+}
+
+int marfa_batch_info(marfa_batch *b, int32_t job, int32_t *nlev, int32_t *nrec, float **records_dev, float *last_ms) {
+    if (!b) return MARFA_E_ARG;
+    if (last_ms) *last_ms = b->last_ms;
+    if (job < 0) return 0;
+    if (job >= (int32_t)b->jobs.size()) return bset_err(b, MARFA_E_ARG, "no such job");
+    const auto &J = b->jobs[job];
+    if (nlev) *nlev = (int32_t)J.L.P.size();
+    if (nrec) *nrec = J.W.nrec;
+    if (records_dev) *records_dev = (float *)J.ctx->out.p;
+    return 0;
+}
+
+int marfa_batch_records(marfa_batch *b, int32_t job, float *out_host) {
+    if (!b || !out_host) return MARFA_E_ARG;
+    if (job < 0 || job >= (int32_t)b->jobs.size()) return bset_err(b, MARFA_E_ARG, "no such job");
+    (void)hipSetDevice(b->device);
+    for (auto *l : b->lanes) BHIPCHK(hipStreamSynchronize(l->stream));
+    BHIPCHK(hipDeviceSynchronize());
+    const auto &J = b->jobs[job];
+    const size_t total = J.L.P.size() * (size_t)J.W.nrec * NT;
+    if (total) BHIPCHK(hipMemcpy(out_host, J.ctx->out.p, total * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 

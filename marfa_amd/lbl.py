@@ -65,6 +65,17 @@ def library() -> C.CDLL:
     L.marfa_pt_tables.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, C.c_int32]
     L.marfa_pt_tables_adjoint.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
     L.marfa_pt_tables_f64.argtypes = [vp, C.POINTER(_Job), vp, vp]
+    L.marfa_batch_create.argtypes = [C.c_int, C.c_int32, C.c_double, C.POINTER(C.c_void_p)]
+    L.marfa_batch_destroy.argtypes = [vp]
+    L.marfa_batch_last_error.argtypes = [vp]
+    L.marfa_batch_last_error.restype = C.c_char_p
+    L.marfa_batch_set_tips.argtypes = [vp, C.c_int32, C.c_int32, fp]
+    L.marfa_batch_add_job.argtypes = [vp, C.POINTER(_Job), C.c_int64] + [vp] * 8 + [C.c_int32, fp, fp, fp, i32p]
+    L.marfa_batch_tune.argtypes = [vp, vp, vp]
+    L.marfa_batch_capture.argtypes = [vp]
+    L.marfa_batch_launch.argtypes = [vp, vp, C.c_int32]
+    L.marfa_batch_info.argtypes = [vp, C.c_int32, i32p, i32p, C.POINTER(C.c_void_p), C.POINTER(C.c_float)]
+    L.marfa_batch_records.argtypes = [vp, C.c_int32, fp]
     L.marfa_count_samples.argtypes = [vp, C.POINTER(_Job), i64p]
     L.marfa_grid_sums.argtypes = [vp, C.POINTER(_Job), C.c_int32, C.c_int32, dp, fp]
     L.marfa_last_timings.argtypes = [vp, fp]
@@ -76,7 +87,9 @@ def library() -> C.CDLL:
 EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_version",
            "marfa_set_tips", "marfa_set_lines", "marfa_set_levels", "marfa_windows",
            "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint", "marfa_count_samples",
-           "marfa_grid_sums", "marfa_last_timings")
+           "marfa_grid_sums", "marfa_last_timings", "marfa_batch_create", "marfa_batch_destroy",
+           "marfa_batch_last_error", "marfa_batch_set_tips", "marfa_batch_add_job", "marfa_batch_tune",
+           "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records")
 
 
 def _p(a):
@@ -237,3 +250,77 @@ class LBLContext:
         t = np.zeros(5, np.float32)
         self._check(self.lib.marfa_last_timings(self.h, _p(t)))
         return dict(zip(("spectroscopy", "sort", "coarse", "fine", "total"), t.tolist()))
+
+
+NVARIANTS = 6
+VARIANTS = [(512, True), (1024, True), (2048, True), (512, False), (1024, False), (2048, False)]
+
+
+class Batch:
+    """Many independent jobs recorded into one hipGraph (marfa_batch_* in include/marfa.h)."""
+
+    def __init__(self, device: int = 0, lanes: int = 4, lane_budget_gb: float = 0.0):
+        self.lib = library()
+        h = C.c_void_p()
+        rc = self.lib.marfa_batch_create(device, lanes, lane_budget_gb, C.byref(h))
+        if rc:
+            raise MarfaError(rc, "could not create a batch (is a GPU visible?)")
+        self.h = h
+        self.shapes = []
+
+    def _check(self, rc):
+        if rc:
+            raise MarfaError(rc, self.lib.marfa_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.marfa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tips(self, tips):
+        t = np.ascontiguousarray(tips, np.float32)
+        self._check(self.lib.marfa_batch_set_tips(self.h, t.shape[0], t.shape[1], _p(t)))
+
+    def add_job(self, job, precision=FP32K) -> int:
+        L, A = job.lines, job.atm
+        arrs = [np.ascontiguousarray(L.nu, np.float64)]
+        for f in ("S", "gair", "gself", "elow", "nT", "delta"):
+            arrs.append(np.ascontiguousarray(getattr(L, f), np.float32))
+        arrs.append(np.ascontiguousarray(L.molIso, np.int32))
+        lv = [np.ascontiguousarray(x, np.float32) for x in (A.P, A.T, A.rho)]
+        jid = C.c_int32()
+        j = JobSpec.from_job(job, precision)._c()
+        self._check(self.lib.marfa_batch_add_job(self.h, C.byref(j), len(arrs[0]), *[_p(a) for a in arrs],
+                                                 len(lv[0]), *[_p(x) for x in lv], C.byref(jid)))
+        nlev, nrec = C.c_int32(), C.c_int32()
+        self._check(self.lib.marfa_batch_info(self.h, jid.value, C.byref(nlev), C.byref(nrec), None, None))
+        self.shapes.append((nlev.value, nrec.value, NT))
+        return jid.value
+
+    def tune(self):
+        n = len(self.shapes)
+        v = np.zeros(max(n, 1), np.int32)
+        t = np.zeros((max(n, 1), NVARIANTS), np.float32)
+        self._check(self.lib.marfa_batch_tune(self.h, _p(v), _p(t)))
+        return v[:n], t[:n]
+
+    def capture(self):
+        self._check(self.lib.marfa_batch_capture(self.h))
+
+    def launch(self, stream: int = 0, sync: bool = True) -> float:
+        """One replay of the graph; returns its duration [ms] when sync."""
+        self._check(self.lib.marfa_batch_launch(self.h, C.c_void_p(stream) if stream else None, 1 if sync else 0))
+        ms = C.c_float()
+        self._check(self.lib.marfa_batch_info(self.h, -1, None, None, None, C.byref(ms)))
+        return ms.value
+
+    def records(self, job: int) -> np.ndarray:
+        out = np.zeros(self.shapes[job], np.float32)
+        self._check(self.lib.marfa_batch_records(self.h, job, _p(out)))
+        return out
