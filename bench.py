@@ -195,6 +195,70 @@ def batch_bench(steps=2, lanes=4, check_jobs=24):
                                      "max": float(e.max())} if errs else None}
 
 
+def large_bench(levels_checked=(0, 49, 99)):
+    """8,000,000 H2O-like lines over 10-20,000 cm-1, 100 levels (P 100 -> 1e-5 atm, T 750 ->
+    160 K), cutoff 250, FP32K: forward and adjoint (grid sums recomputed per record tile, never
+    stored) of sum (log k - log k_hat)^2, k_hat from S and gamma_air perturbed by U(-5%, 5%);
+    points/s, kernel roofline fractions, and the error against the FP64 oracle on the
+    10000-10100 cm-1 window at three levels."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from marfa_amd import synth, lbl
+    from marfa_amd.autograd import PTTableModel, PTTables, spectral_loss
+    t0 = time.time()
+    job = synth.config5()
+    tips = synth.load_tips()
+    model = PTTableModel(job, tips, precision=lbl.FP32K)
+    samples, level0 = model.ctx.count_samples(model.spec, coarse=True)
+    base = list(model.params())
+    rng = np.random.default_rng(synth.SEED_BASE + 5)
+    pert = [b.clone() for b in base]
+    for k in (0, 1):
+        pert[k] = pert[k] * torch.as_tensor(1 + rng.uniform(-0.05, 0.05, tuple(pert[k].shape)),
+                                            dtype=torch.float32, device="cuda")
+    model.set_params(*pert)
+    target = model.records()
+    tp = [b.clone().requires_grad_(True) for b in base]
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    rec = PTTables.apply(*tp, model)
+    e1.record()
+    fwd_phases = model.ctx.timings()
+    loss = spectral_loss(rec, target)
+    loss.backward()
+    e2.record()
+    torch.cuda.synchronize()
+    adj_phases = model.ctx.timings()
+    fwd_ms, adj_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    nlev, nrec = model.shape[0], model.shape[1]
+    points = nlev * nrec * lbl.NT
+    # error on the 10000-10100 cm-1 window
+    _, _, _, org = model.ctx.windows(model.spec)
+    rs = [int(r) for r in np.nonzero((org >= 10000.0) & (org < 10100.0))[0]]
+    oj = po.OracleJob(job, tips)
+    errs = []
+    recs = rec.detach()
+    for lev in levels_checked:
+        for r in rs:
+            ref, _ = oj.record(po.F64, lev, r)
+            g = recs[lev, r].cpu().numpy().astype(np.float64)
+            m = np.abs(ref) > np.finfo(np.float32).tiny
+            errs.append(np.abs(g[m] - ref[m]) / np.abs(ref[m]))
+    e = np.concatenate(errs)
+    model.close()
+    coarse_tf = FLOP_PER_SAMPLE * level0 / (fwd_phases["coarse"] / 1e3) / 1e12
+    fine_tf = (FLOP_PER_SAMPLE * (samples - level0) + FLOP_PER_RECORD_CASCADE * nlev * nrec) / (fwd_phases["fine"] / 1e3) / 1e12
+    return {"workload": "8,000,000 lines 10-20000 cm-1, 100 levels P 100->1e-5 atm T 750->160 K, cutoff 250, "
+                        "no chi, FP32K", "points": int(points), "samples": int(samples),
+            "fwd_ms": fwd_ms, "adjoint_ms": adj_ms, "pts_per_s": points / (fwd_ms / 1e3),
+            "fwd_phases_ms": fwd_phases, "adjoint_phases_ms": adj_phases,
+            "roofline_frac_fp32": {"coarse_kernel": coarse_tf / PEAK["fp32"], "fine_kernel": fine_tf / PEAK["fp32"]},
+            "window_10000_10100_vs_fp64": {"levels": list(levels_checked), "records": len(rs),
+                                           "max_rel_err": float(e.max()), "rms_rel_err": float(np.sqrt(np.mean(e ** 2)))},
+            "setup_s": time.time() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +270,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 drop-in timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-job batched graph timing")
+    ap.add_argument("--no-large", action="store_true", help="skip the 8M-line, 100-level forward + adjoint")
     ap.add_argument("--cpu-samples", type=float, default=1.0e9,
                     help="profile samples in the reference CPU sample (~17 s at 5.8e7 samples/s)")
     ap.add_argument("--nlines", type=int, default=2_000_000)
@@ -325,6 +390,10 @@ def main():
     if not args.no_batch:
         result["batched"] = batch_bench()
         log(f"[bench] batched {result['batched']}")
+
+    if not args.no_large:
+        result["large"] = large_bench()
+        log(f"[bench] large {result['large']}")
 
     if not args.no_adjoint:
         result["adjoint"] = adjoint_bench()
