@@ -9,7 +9,11 @@
 // candidate range, and the reference's walk is inverted in closed form for the exact
 // test.  Grids 5..9 and the fine points read their centre lines from an LDS copy of the
 // tile's reach (u and sorted index, with a local bucket table); grids 1..4, whose reach
-// spans several tiles, and the wing lines read the level's sorted arrays directly.  For a line at u = nu* - a (src/LineGridCalc.f90:191) the left walk of centerLBL
+// spans several tiles, and the wing lines read the level's sorted arrays directly.  In the
+// FP32 form the next line's shape parameters are loaded while the current line is
+// evaluated.
+//
+// For a line at u = nu* - a (src/LineGridCalc.f90:191) the left walk of centerLBL
 // (:201-331) covers, on grid k, the cells (2 e_{k-1}, e_k] with e_k = max(0, floor(u 2^k) - 1);
 // the right walk (:343-484) the mirror image in v = 10 - u; the core fine points are
 // [4 e_9 + 2, floor(u/H) + 1] (:332-339) and [floor(u/H) + 2, NT - 4 e'_9] (:485-490).
@@ -86,6 +90,7 @@ template <typename R>
 struct GlobalSrc {
     const FineCtx<R> &f;
     const ShapeCtl &ctl;
+    static constexpr bool kPrefetch = false;  // most candidates are rejected before their shape is read
     __device__ GlobalSrc(const FineCtx<R> &f_) : f(f_), ctl(f_.ctl) {}
     __device__ inline void range(double u0, double u1, int &j0, int &j1) const { cand(f, u0, u1, j0, j1); }
     __device__ inline int cls(int j, double &u) const { return classify(f, j, u); }
@@ -110,6 +115,9 @@ struct CentreLds {
     double ustart;
     const LineShape<R> *shp;
     ShapeCtl ctl;
+    // every staged line is a centre line; the FP64 form passes the shape to an out-of-line
+    // call, where a register copy would go through scratch
+    static constexpr bool kPrefetch = sizeof(R) == 4;
     __device__ inline void range(double u0, double u1, int &j0, int &j1) const {
         const double x0 = (u0 - ustart) * BUCKETS_PER_CM, x1 = (u1 - ustart) * BUCKETS_PER_CM;
         const int b0 = (int)fmin(fmax(floor(x0) - 1.0, 0.0), (double)STAGE_NB);
@@ -123,6 +131,24 @@ struct CentreLds {
     }
     __device__ inline const LineShape<R> &shape(int j) const { return shp[sj[j]]; }
 };
+
+// Visit candidates j = j0, j0 + step, ... < j1 with their line parameters.  For the staged
+// source (only centre lines) the next line's parameters are loaded before the current line
+// is evaluated, so that the global load overlaps the evaluation.
+template <typename R, typename Src, typename F>
+__device__ inline void for_lines(const Src &src, int j0, int j1, int step, F &&f) {
+    if constexpr (Src::kPrefetch) {
+        if (j0 >= j1) return;
+        LineShape<R> nxt = src.shape(j0);
+        for (int j = j0; j < j1; j += step) {
+            const LineShape<R> cur = nxt;
+            if (j + step < j1) nxt = src.shape(j + step);
+            f(j, cur);
+        }
+    } else {
+        for (int j = j0; j < j1; j += step) f(j, src.shape(j));
+    }
+}
 
 // Sums at one position node of grid k: global position q in steps of H_k/2.  Even q is the
 // right edge (L) of cell q/2 -> aL and the left edge (P) of cell q/2+1 -> aP; odd q is the
@@ -140,18 +166,18 @@ __device__ inline void gather_level_centre(const Src &src, int k, int q, bool ne
         const double ulo = nlo * Hk, uhi = (nhi + 1) * Hk;
         int j0, j1;
         src.range(ulo, uhi, j0, j1);
-        for (int j = j0 + jfirst; j < j1; j += jstep) {
+        for_lines<R>(src, j0 + jfirst, j1, jstep, [&](int j, const LineShape<R> &p) {
             double u;
-            if (src.cls(j, u) != 2 || u < ulo || !(u < uhi)) continue;
+            if (src.cls(j, u) != 2 || u < ulo || !(u < uhi)) return;
             const int n = (int)floor(__builtin_ldexp(u, k));
             const int lo_q = 4 * max(0, (n >> 1) - 1), hi_q = 2 * (n - 1);
-            if (q < lo_q || q > hi_q) continue;
+            if (q < lo_q || q > hi_q) return;
             const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
-            if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(u - xq, src.shape(j), src.ctl);
+            if (!toL && !toP) return;
+            const R val = shape_eval<R, true>(u - xq, p, src.ctl);
             if (toL) aL.add(val);
             if (toP) aP.add(val);
-        }
+        });
     }
     // centre lines, right walk: floor(v 2^k) in [nlo, nhi] for q' = 2 NT_k - q, v = 10 - u
     {
@@ -160,20 +186,20 @@ __device__ inline void gather_level_centre(const Src &src, int k, int q, bool ne
         const double vlo = nlo * Hk, vhi = (nhi + 1) * Hk;
         int j0, j1;
         src.range(10.0 - vhi - 1e-9, 10.0 - vlo + 1e-9, j0, j1);
-        for (int j = j0 + jfirst; j < j1; j += jstep) {
+        for_lines<R>(src, j0 + jfirst, j1, jstep, [&](int j, const LineShape<R> &p) {
             double u;
-            if (src.cls(j, u) != 2) continue;
+            if (src.cls(j, u) != 2) return;
             const double v = 10.0 - u;
-            if (v < vlo || !(v < vhi)) continue;
+            if (v < vlo || !(v < vhi)) return;
             const int n = (int)floor(__builtin_ldexp(v, k));
             const int lo_q = 2 * NTk - 2 * (n - 1), hi_q = 2 * NTk - 4 * max(0, (n >> 1) - 1);
-            if (q < lo_q || q > hi_q) continue;
+            if (q < lo_q || q > hi_q) return;
             const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
-            if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(v - qp * hq, src.shape(j), src.ctl);
+            if (!toL && !toP) return;
+            const R val = shape_eval<R, true>(v - qp * hq, p, src.ctl);
             if (toL) aL.add(val);
             if (toP) aP.add(val);
-        }
+        });
     }
 }
 
@@ -253,12 +279,12 @@ __device__ inline void gather_fine_centre(const Src &src, int i, Acc<R> &acc) {
         const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
         int j0, j1;
         src.range(10.0 - vhi - 1e-9, uhi, j0, j1);
-        for (int j = j0; j < j1; j++) {
+        for_lines<R>(src, j0, j1, 1, [&](int j, const LineShape<R> &p) {
             double u;
-            if (src.cls(j, u) != 2) continue;
-            if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) continue;
-            acc.add(shape_eval<R, true>(u - xi, src.shape(j), src.ctl));
-        }
+            if (src.cls(j, u) != 2) return;
+            if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) return;
+            acc.add(shape_eval<R, true>(u - xi, p, src.ctl));
+        });
     }
 }
 
@@ -455,18 +481,11 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t) {
     }
 }
 
-// CAP: centre lines staged in LDS per tile (LDS 27.5 KB + 12 B x CAP: occupancy against
-// staging); XCD: contiguous record ranges per XCD (xcd_item) or plain dispatch order.
-template <typename R, int CAP = STAGE_CAP, bool XCD = true>
-__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
-    __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
-    __shared__ double s_u[CAP];           // staged centre lines
-    __shared__ int32_t s_j[CAP];
-    __shared__ int32_t s_b[STAGE_NB + 1];
-    __shared__ int s_cnt[BLOCK / 64];
-    const int nitems = a.nlev * a.nrec * TILES;
-    const int w = XCD ? xcd_item(nitems) : (int)blockIdx.x;
-    if (w >= nitems) return;
+// One (record, tile) work item w by node-owner gathers: grid 1..9 sums into g, cascade,
+// fine points in registers, record write.  LDS: g[NACC], staging arrays of capacity cap.
+template <typename R>
+__device__ inline void fine_tile(const KArgs &a, int w, double *g, double *s_u, int32_t *s_j, int32_t *s_b,
+                                 int *s_cnt, int cap) {
     const int r = w / TILES, t = w % TILES;
     const int lev = r / a.nrec, s = r % a.nrec;
     const double sc = a.sscale[a.lev0 + lev];
@@ -475,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     tile_ctx<R>(a, lev, s, f);
     const GlobalSrc<R> gsrc(f);
     CentreLds<R> lsrc;
-    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc, CAP);
+    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc, cap);
     tile_level_sums<R>(gsrc, lsrc, staged, t, g);
 
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
@@ -531,6 +550,21 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
         if (a.out64) a.out64[ro] = cz[0] * mul;
         else a.out[ro] = (float)(cz[0] * mul);
     }
+}
+
+// CAP: centre lines staged in LDS per tile (LDS 27.5 KB + 12 B x CAP: occupancy against
+// staging); XCD: contiguous record ranges per XCD (xcd_item) or plain dispatch order.
+template <typename R, int CAP = STAGE_CAP, bool XCD = true>
+__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
+    __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
+    __shared__ double s_u[CAP];           // staged centre lines
+    __shared__ int32_t s_j[CAP];
+    __shared__ int32_t s_b[STAGE_NB + 1];
+    __shared__ int s_cnt[BLOCK / 64];
+    const int nitems = a.nlev * a.nrec * TILES;
+    const int w = XCD ? xcd_item(nitems) : (int)blockIdx.x;
+    if (w >= nitems) return;
+    fine_tile<R>(a, w, g, s_u, s_j, s_b, s_cnt, CAP);
 }
 
 // ---- adjoint of the deposit + cascade of one (record, tile)
