@@ -3,7 +3,7 @@
 
 Workload (default): 2,000,000 synthetic CO2-like lines over 500-10,000 cm-1 in 20
 Gaussian bands, 60 levels (P 90 -> 1e-4 atm, T 735 -> 170 K), cutoff 125 cm-1, Pollack
-chi factor, FP32 samples with Kahan-compensated sums and FP64 offsets.  One step = every
+chi factor, FP32 samples with FP64 sums and FP64 offsets.  One step = every
 PT-table record of every level (60 x 950 records x 20,481 points), inputs resident in HBM,
 records left in HBM.
 
@@ -366,7 +366,7 @@ def main():
         "config": {"workload": "2,000,000 lines 500-10000 cm-1 (20 Gaussian bands), 60 levels P 90->1e-4 atm "
                                "T 735->170 K, cutoff 125 cm-1, Pollack chi, Voigt",
                    "lines": len(job.lines), "levels": nlev, "records_per_level": nrec, "points": points,
-                   "samples": samples, "form": "FP32+Kahan, FP64 offsets" if prec == lbl.FP32K else "FP64 plain"},
+                   "samples": samples, "form": "FP32 samples, FP64 sums and offsets" if prec == lbl.FP32K else "FP64 plain"},
         "samples_per_s": samples / dt,
         "whole_path_tflops": whole,
         "phases_ms": dict(zip(("spectroscopy", "sort", "coarse", "fine", "total"), phase.tolist())),

@@ -41,7 +41,7 @@ extern "C" {
 
 /* precision forms */
 #define MARFA_FP64 0  /* plain drop-in: the reference's algorithm in FP64 */
-#define MARFA_FP32K 1 /* FP32 profiles + Kahan-compensated FP32 sums, FP64 offsets */
+#define MARFA_FP32K 1 /* FP32 profiles, compensated FP32 or FP64 sums, FP64 offsets */
 
 /* shapes (src/Shapes.f90) and chi factors (src/ChiFactors.f90) */
 #define MARFA_SHAPE_VOIGT 0    /* voigt, Shapes.f90:22 */

@@ -396,25 +396,14 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
 }
 
 // ---------------------------------------------------------------- accumulators
-// FP64: plain sums.  FP32: Kahan-compensated pairs (value s, compensation c; the
-// compensated sum is s - c).
-template <typename R> struct Acc;
-template <> struct Acc<double> {
+// Node and record sums are FP64 in both precision forms: an FP32 sample enters an FP64 sum
+// exactly, which is more accurate than a compensated FP32 pair held in the same two
+// registers and takes fewer operations per add.
+struct Acc {
     double s;
     __device__ inline void zero() { s = 0.0; }
     __device__ inline void add(double y) { s += y; }
     __device__ inline double value() const { return s; }
-};
-template <> struct Acc<float> {
-    float s, c;
-    __device__ inline void zero() { s = 0.f; c = 0.f; }
-    __device__ inline void add(float y) {
-        float yy = y - c;
-        float t = s + yy;
-        c = (t - s) - yy;
-        s = t;
-    }
-    __device__ inline double value() const { return (double)s - (double)c; }
 };
 
 }  // namespace marfa

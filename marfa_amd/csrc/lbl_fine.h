@@ -156,7 +156,7 @@ __device__ inline void for_lines(const Src &src, int j0, int j1, int step, F &&f
 // Centre lines first (left walk, then right walk), then wing lines, each in sorted order.
 template <typename R, typename Src>
 __device__ inline void gather_level_centre(const Src &src, int k, int q, bool needL, bool needP,
-                                           int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+                                           int jfirst, int jstep, Acc &aL, Acc &aP) {
     const int NTk = 10 << k;
     const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
     const bool odd = q & 1;
@@ -205,7 +205,7 @@ __device__ inline void gather_level_centre(const Src &src, int k, int q, bool ne
 
 template <typename R, typename Src>
 __device__ inline void gather_level_wings(const Src &src, int k, int q, bool needL, bool needP,
-                                          int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+                                          int jfirst, int jstep, Acc &aL, Acc &aP) {
     const int NTk = 10 << k;
     const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
     // wing lines closer than H_{k-1} to the left edge: cells 1 (m == k) and 2 (m >= k)
@@ -262,14 +262,14 @@ __device__ inline void gather_level_wings(const Src &src, int k, int q, bool nee
 
 template <typename R, typename Src>
 __device__ inline void gather_level_node(const Src &src, int k, int q, bool needL, bool needP,
-                                         int jfirst, int jstep, Acc<R> &aL, Acc<R> &aP) {
+                                         int jfirst, int jstep, Acc &aL, Acc &aP) {
     gather_level_centre<R>(src, k, q, needL, needP, jfirst, jstep, aL, aP);
     gather_level_wings<R>(src, k, q, needL, needP, jfirst, jstep, aL, aP);
 }
 
 // direct deposits at fine point i (1-based record index, position (i-1) H): centre lines
 template <typename R, typename Src>
-__device__ inline void gather_fine_centre(const Src &src, int i, Acc<R> &acc) {
+__device__ inline void gather_fine_centre(const Src &src, int i, Acc &acc) {
     const double xi = (double)(i - 1) * HF;
     {   // centre lines whose core reaches point i.  Left part (the line at or right of the
         // point): u in [(i-1)H, (floor((i-2)/4) + 2) H9).  Right part (the line left of the
@@ -290,7 +290,7 @@ __device__ inline void gather_fine_centre(const Src &src, int i, Acc<R> &acc) {
 
 // direct deposits of wing lines at the first / last fine points of the record
 template <typename R, typename Src>
-__device__ inline void gather_fine_wings(const Src &src, int i, Acc<R> &acc) {
+__device__ inline void gather_fine_wings(const Src &src, int i, Acc &acc) {
     if (i <= 5) {  // leftLBL :107-112 (d < H9)
         int j0, j1;
         src.range(-H9, 0.0, j0, j1);
@@ -415,7 +415,7 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
         while (k < 4 && nd >= lvl_node_off(k + 1)) k++;
         const int qloc = nd - lvl_node_off(k), span = 2 << k;
         const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-        Acc<R> aL, aP;
+        Acc aL, aP;
         aL.zero();
         aP.zero();
         gather_level_node<R>(gsrc, k, t * span + qloc, nL, nP, lane, 64, aL, aP);
@@ -431,7 +431,7 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
         while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
         const int qloc = nd - lvl_node_off(k), span = 2 << k, q = t * span + qloc;
         const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-        Acc<R> aL, aP;
+        Acc aL, aP;
         aL.zero();
         aP.zero();
         if (staged) gather_level_centre<R>(lsrc, k, q, nL, nP, 0, 1, aL, aP);
@@ -521,7 +521,7 @@ __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *s_u, 
 #pragma unroll 1
         for (int e = 0; e < 4; e++) {
             const int il = 4 * c + e, i = 2048 * t + 2 + il;
-            Acc<R> acc;
+            Acc acc;
             acc.zero();
             if (staged) gather_fine_centre<R>(lsrc, i, acc);
             else gather_fine_centre<R>(gsrc, i, acc);

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
     const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
     const double sc = a.sscale[a.lev0 + lev];
 
-    Acc<R> aR, aP[10], aM[10], aL[10];
+    Acc aR, aP[10], aM[10], aL[10];
     aR.zero();
 #pragma unroll
     for (int c = 0; c < 10; c++) { aP[c].zero(); aM[c].zero(); aL[c].zero(); }
@@ -235,6 +235,26 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
         const bool use = level0_masks(nus, org, cut, mP, mM, mL);
         if (!use) continue;
         const LineShape<R> p = make_line<R>(a.gD[base + i], a.gL[base + i], a.S[base + i] * sc);
+        // a line at least 1 cm-1 outside the subinterval deposits into every level-0 node
+        // (full masks); when even its nearest node, |X| >= dmin, is on the Lorentz branch
+        // (|X| xs >= margin as shape_eval tests it; rounding is monotone, so every node is),
+        // the nodes are summed without masks or per-sample branch tests
+        const double dmin = -u >= 1.0 ? -u : u - 10.0;
+        const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
+        if (dmin >= 1.0 && (ctl.shape_id == 1 || (R)dmin * p.xs >= margin)) {
+#pragma unroll
+            for (int q = 0; q <= 20; q++) {
+                const R val = chilor<R, CHI>((R)(u - 0.5 * q), p, ctl);
+                if (q == 0) aR.add(val);
+                if (q & 1) {
+                    aM[q >> 1].add(val);
+                } else {
+                    if (q <= 18) aP[q >> 1].add(val);
+                    if (q >= 2) aL[(q >> 1) - 1].add(val);
+                }
+            }
+            continue;
+        }
         const uint32_t need = mP | mM | mL | 1u;
 #pragma unroll
         for (int q = 0; q <= 20; q++) {
