@@ -71,6 +71,28 @@ __device__ inline R chi_factor(int chi, R ax, R B1, R B2) {
     return R(1);
 }
 
+// chi_factor with the band chosen by selects instead of branches for chi 1 and 2, whose
+// bands all read coef * exp(-(rate |X|)): each band is evaluated with chi_factor's
+// operations, so the values are the same; Perrin (chi 3) keeps chi_factor
+template <typename R>
+__device__ inline R chi_factor_sel(int chi, R ax, R B1, R B2) {
+    if (chi == 1) {
+        const bool b1 = ax <= R(150), b2 = ax <= R(300);
+        const R coef = b1 ? R(1.084f) : b2 ? R(0.208f) : R(0.025f);
+        const R rate = b1 ? R(0.027f) : b2 ? R(0.016f) : R(0.009f);
+        const R v = coef * dexp<R>(-(rate * ax));
+        return ax > R(3) ? v : R(1);
+    }
+    if (chi == 2) {
+        const bool b1 = ax < R(10);
+        const R coef = b1 ? R(1.35f) : R(0.614f);
+        const R rate = b1 ? R(0.1) : R(1.0 / 47.0);
+        const R v = coef * dexp<R>(-ax * rate);
+        return ax >= R(3) ? v : R(1);
+    }
+    return chi_factor<R>(chi, ax, B1, B2);
+}
+
 // ---------------------------------------------------------------- per-line state
 // Derived once per (line, level) visit; R is the sample precision.
 template <typename R>
@@ -141,8 +163,8 @@ __device__ inline R chi_sel(const ShapeCtl &c, R ax) {
         return R(1);
     } else {
         if (c.mol != 2) return R(1);
-        if constexpr (sizeof(R) == 8) return chi_factor<double>(CHI, ax, c.B1d, c.B2d);
-        else return chi_factor<float>(CHI, ax, c.B1, c.B2);
+        if constexpr (sizeof(R) == 8) return chi_factor_sel<double>(CHI, ax, c.B1d, c.B2d);
+        else return chi_factor_sel<float>(CHI, ax, c.B1, c.B2);
     }
 }
 
@@ -231,6 +253,14 @@ template <typename R, int CHI = -1>
 __device__ inline R chilor(R X, const LineShape<R> &p, const ShapeCtl &c) {
     R lor = p.lorc * recip<R>(X * X + p.gl2);
     if constexpr (CHI != 0) lor *= chi_sel<R, CHI>(c, X < R(0) ? -X : X);
+    return lor;
+}
+
+// the same from ax = |X| (X enters only as X^2 and |X|)
+template <typename R, int CHI = -1>
+__device__ inline R chilor_abs(R ax, const LineShape<R> &p, const ShapeCtl &c) {
+    R lor = p.lorc * recip<R>(ax * ax + p.gl2);
+    if constexpr (CHI != 0) lor *= chi_sel<R, CHI>(c, ax);
     return lor;
 }
 

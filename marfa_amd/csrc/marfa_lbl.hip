@@ -209,8 +209,10 @@ __device__ inline bool level0_masks(double nus, double org, double cut, uint32_t
     return true;
 }
 
+// FP32 form: registers capped for 3 waves per SIMD (fits without spills); the FP64 form's
+// larger state would spill under that cap and runs at 2
 template <typename R, int CHI>
-__global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 1) void coarse_kernel(KArgs a) {
     const int nitems = a.nlev * a.nrec;
     const int r = xcd_item(nitems);
     if (r >= nitems) return;
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void coarse_kernel(KArgs a) {
         if (dmin >= 1.0 && (ctl.shape_id == 1 || (R)dmin * p.xs >= margin)) {
 #pragma unroll
             for (int q = 0; q <= 20; q++) {
-                const R val = chilor<R, CHI>((R)(u - 0.5 * q), p, ctl);
+                const R val = chilor_abs<R, CHI>((R)fabs(u - 0.5 * q), p, ctl);
                 if (q == 0) aR.add(val);
                 if (q & 1) {
                     aM[q >> 1].add(val);
