@@ -118,12 +118,24 @@ struct CentreLds {
     // every staged line is a centre line; the FP64 form passes the shape to an out-of-line
     // call, where a register copy would go through scratch
     static constexpr bool kPrefetch = sizeof(R) == 4;
+    // the local buckets give a superset; the staged u are sorted, so two binary searches
+    // inside it give exactly the lines with u in [u0, u1)
     __device__ inline void range(double u0, double u1, int &j0, int &j1) const {
         const double x0 = (u0 - ustart) * BUCKETS_PER_CM, x1 = (u1 - ustart) * BUCKETS_PER_CM;
         const int b0 = (int)fmin(fmax(floor(x0) - 1.0, 0.0), (double)STAGE_NB);
         const int b1 = (int)fmin(fmax(floor(x1) + 2.0, 0.0), (double)STAGE_NB);
-        j0 = sb[b0];
-        j1 = b1 > b0 ? sb[b1] : j0;
+        const int a0 = sb[b0], a1 = b1 > b0 ? sb[b1] : a0;
+        j0 = lower(a0, a1, u0);
+        j1 = lower(j0, a1, u1);
+    }
+    // first staged index in [lo, hi) with u >= x (hi when none)
+    __device__ inline int lower(int lo, int hi, double x) const {
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (su[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
     }
     __device__ inline int cls(int j, double &u) const {
         u = su[j];
