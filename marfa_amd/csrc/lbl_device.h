@@ -301,9 +301,21 @@ __device__ inline R voigt_regions(double X64, const LineShape<R> &p, const Shape
     return doppler<R>(X, p);
 }
 
-// out-of-line copy for kernels where the non-Lorentz regions are rare (level-0 deposits)
+// out-of-line copy for kernels where the non-Lorentz regions are rare (level-0 deposits).
+// The line's fields travel as scalar arguments (VGPRs): a LineShape passed by reference
+// needs an address, and the caller would then store every line's parameters to scratch
+// memory, whether or not the call is taken.
 template <typename R>
-__device__ __noinline__ R voigt_slow(double X64, const LineShape<R> &p, const ShapeCtl &c) {
+__device__ __noinline__ R voigt_slow(double X64, double xs64, double vy64, R lorc, R gl2, R normc, R dopa, R igd,
+                                     const ShapeCtl &c) {
+    LineShape<R> p;
+    p.xs64 = xs64;
+    p.vy64 = vy64;
+    p.lorc = lorc;
+    p.gl2 = gl2;
+    p.normc = normc;
+    p.dopa = dopa;
+    p.igd = igd;
     return voigt_regions<R>(X64, p, c);
 }
 
@@ -320,7 +332,7 @@ __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl
     const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
     if (ax * p.xs >= margin) return chilor<R, CHI>(X, p, c);
     if constexpr (NEAR && sizeof(R) == 4) return voigt_regions<R>(X64, p, c);
-    else return voigt_slow<R>(X64, p, c);
+    else return voigt_slow<R>(X64, p.xs64, p.vy64, p.lorc, p.gl2, p.normc, p.dopa, p.igd, c);
 }
 
 // ---------------------------------------------------------------- derivatives (adjoint)

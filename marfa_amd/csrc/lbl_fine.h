@@ -162,56 +162,51 @@ __device__ inline void for_lines(const Src &src, int j0, int j1, int step, F &&f
     }
 }
 
-// Sums at one position node of grid k: global position q in steps of H_k/2.  Even q is the
-// right edge (L) of cell q/2 -> aL and the left edge (P) of cell q/2+1 -> aP; odd q is the
-// midpoint of cell (q+1)/2 -> aL.  needL / needP: the roles this tile owns.
-// Centre lines first (left walk, then right walk), then wing lines, each in sorted order.
-template <typename R, typename Src>
-__device__ inline void gather_level_centre(const Src &src, int k, int q, bool needL, bool needP,
-                                           int jfirst, int jstep, Acc &aL, Acc &aP) {
-    const int NTk = 10 << k;
-    const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
-    const bool odd = q & 1;
-    // centre lines, left walk: floor(u 2^k) in [nlo, nhi]
-    {
-        const int nlo = max(2, ((q + 1) >> 1) + 1), nhi = 2 * (q >> 2) + 3;
-        const double ulo = nlo * Hk, uhi = (nhi + 1) * Hk;
-        int j0, j1;
-        src.range(ulo, uhi, j0, j1);
-        for_lines<R>(src, j0 + jfirst, j1, jstep, [&](int j, const LineShape<R> &p) {
-            double u;
-            if (src.cls(j, u) != 2 || u < ulo || !(u < uhi)) return;
-            const int n = (int)floor(__builtin_ldexp(u, k));
-            const int lo_q = 4 * max(0, (n >> 1) - 1), hi_q = 2 * (n - 1);
-            if (q < lo_q || q > hi_q) return;
-            const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
-            if (!toL && !toP) return;
-            const R val = shape_eval<R, true>(u - xq, p, src.ctl);
-            if (toL) aL.add(val);
-            if (toP) aP.add(val);
-        });
-    }
-    // centre lines, right walk: floor(v 2^k) in [nlo, nhi] for q' = 2 NT_k - q, v = 10 - u
-    {
-        const int qp = 2 * NTk - q;
-        const int nlo = max(2, ((qp + 1) >> 1) + 1), nhi = 2 * (qp >> 2) + 3;
-        const double vlo = nlo * Hk, vhi = (nhi + 1) * Hk;
-        int j0, j1;
-        src.range(10.0 - vhi - 1e-9, 10.0 - vlo + 1e-9, j0, j1);
-        for_lines<R>(src, j0 + jfirst, j1, jstep, [&](int j, const LineShape<R> &p) {
-            double u;
-            if (src.cls(j, u) != 2) return;
-            const double v = 10.0 - u;
-            if (v < vlo || !(v < vhi)) return;
-            const int n = (int)floor(__builtin_ldexp(v, k));
-            const int lo_q = 2 * NTk - 2 * (n - 1), hi_q = 2 * NTk - 4 * max(0, (n >> 1) - 1);
-            if (q < lo_q || q > hi_q) return;
-            const bool toL = needL && (odd || q > lo_q), toP = needP && !odd && q < hi_q;
-            if (!toL && !toP) return;
-            const R val = shape_eval<R, true>(v - qp * hq, p, src.ctl);
-            if (toL) aL.add(val);
-            if (toP) aP.add(val);
-        });
+// Centre lines on grid k, by cell pairs.  The left walk of a centre line with
+// n = floor(u 2^k) >= 2 covers cells 2g-1 .. n-1 of grid k, g = n >> 1 (positions
+// q = 4g-4 .. 2n-2 in half cells): the lines of one group g -- u in [g H_{k-1}, (g+1) H_{k-1})
+// -- deposit into the cell pair (2g-1, 2g) and no other centre line's left walk does.  The
+// right walk is the mirror image in v = 10 - u: group g' = NT_k/2 + 1 - g deposits into the
+// same pair.  So the six sums (P, mid, L of both cells) of a pair have one owner, which
+// visits each line of its two groups once and evaluates 3 positions (n even) or 5 (n odd).
+// The lines of a group are split over the owner's L threads (sub-lane sl), each in sorted
+// order.  Slots: 0..2 = P, mid, L of cell 2g-1; 3..5 = the same of cell 2g.
+template <typename R, bool RIGHT, typename Src>
+__device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, double s[6]) {
+    const double Hk = hk(k), hq = 0.5 * Hk;
+    const double wlo = (double)(2 * g) * Hk, whi = (double)(2 * g + 2) * Hk;
+    const double q0 = (double)(4 * g - 4);
+    int j0, j1;
+    if (!RIGHT) src.range(wlo, whi, j0, j1);
+    else src.range(10.0 - whi - 1e-9, 10.0 - wlo + 1e-9, j0, j1);
+    // sums by position e = q - q0 (a2o: e = 2 of lines with n odd, the second cell's edge)
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a2o = 0.0, a3 = 0.0, a4 = 0.0;
+    for_lines<R>(src, j0 + sl, j1, L, [&](int j, const LineShape<R> &p) {
+        double u;
+        if (src.cls(j, u) != 2) return;
+        const double w = RIGHT ? 10.0 - u : u;  // distance walked from the pair's side
+        if (w < wlo || !(w < whi)) return;
+        const bool odd = (int)floor(__builtin_ldexp(w, k)) & 1;
+        const int ne = odd ? 5 : 3;
+        // X = w - q hq at q = q0 + e (the reference's positions: left walk u - x_q, right
+        // walk v - x_q' in mirrored positions)
+#pragma unroll 1
+        for (int e = 0; e < ne; e++) {
+            const double v = (double)shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
+            if (e == 0) a0 += v;
+            else if (e == 1) a1 += v;
+            else if (e == 2) {
+                a2 += v;
+                if (odd) a2o += v;
+            } else if (e == 3) a3 += v;
+            else a4 += v;
+        }
+    });
+    // left walk: e -> P1, M1, L1 (+P2 for n odd), M2, L2; right walk: the mirror image
+    if (!RIGHT) {
+        s[0] += a0; s[1] += a1; s[2] += a2; s[3] += a2o; s[4] += a3; s[5] += a4;
+    } else {
+        s[5] += a0; s[4] += a1; s[3] += a2; s[2] += a2o; s[1] += a3; s[0] += a4;
     }
 }
 
@@ -272,13 +267,6 @@ __device__ inline void gather_level_wings(const Src &src, int k, int q, bool nee
     }
 }
 
-template <typename R, typename Src>
-__device__ inline void gather_level_node(const Src &src, int k, int q, bool needL, bool needP,
-                                         int jfirst, int jstep, Acc &aL, Acc &aP) {
-    gather_level_centre<R>(src, k, q, needL, needP, jfirst, jstep, aL, aP);
-    gather_level_wings<R>(src, k, q, needL, needP, jfirst, jstep, aL, aP);
-}
-
 // direct deposits at fine point i (1-based record index, position (i-1) H): centre lines
 template <typename R, typename Src>
 __device__ inline void gather_fine_centre(const Src &src, int i, Acc &acc) {
@@ -327,19 +315,18 @@ __device__ inline void gather_fine_wings(const Src &src, int i, Acc &acc) {
     }
 }
 
-// offset of grid k's position nodes in the per-tile node list, k = 1..9
-__device__ inline int lvl_node_off(int k) { return (1 << (k + 1)) - 4 + (k - 1); }
 // offset of grid k's P block in the tile's node sums; mid at +2^k, L at +2*2^k
 __device__ inline int lvl_acc_off(int k) { return 3 * ((1 << k) - 2); }
-constexpr int WAVE_NODES = 64;  // position nodes of grids 1..4
 
-__device__ inline void store_level_node(double *g, int k, int t, int qloc, double vL, double vP) {
+// adds a node's wing sums: even qloc is the L edge of tile cell qloc/2 and the P edge of
+// cell qloc/2 + 1, odd qloc the midpoint of cell (qloc+1)/2 (1-based tile cells)
+__device__ inline void add_level_node(double *g, int k, int qloc, double vL, double vP) {
     const int nk = 1 << k, span = 2 << k, accb = lvl_acc_off(k);
     if (qloc & 1) {
-        g[accb + nk + ((qloc + 1) >> 1) - 1] = vL;
+        g[accb + nk + ((qloc + 1) >> 1) - 1] += vL;
     } else {
-        if (qloc > 0) g[accb + 2 * nk + (qloc >> 1) - 1] = vL;
-        if (qloc < span) g[accb + (qloc >> 1)] = vP;
+        if (qloc > 0) g[accb + 2 * nk + (qloc >> 1) - 1] += vL;
+        if (qloc < span) g[accb + (qloc >> 1)] += vP;
     }
 }
 
@@ -414,42 +401,92 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
     return staged;
 }
 
-// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grids 1..4: one
-// wavefront per node, lines split over the lanes, fixed-order reduction (their reach spans
-// several tiles: candidates from global memory); grids 5..9: one thread per node, centre
-// lines from the staged copy.  Ends with a barrier.
+// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grid k has 2^(k-1)
+// cell pairs in the tile; the whole block works on one grid at a time, L = 256 / 2^(k-1)
+// threads per pair, so that every grid's centre lines (2 cm-1 of them: both walks) are
+// spread over all threads.  Grids 1..3 read their lines from the level's sorted arrays
+// (their groups reach up to 2 cm-1 beyond the tile), grids 4..9 from the staged copy.
+// The L partial sums of a pair are reduced in a fixed butterfly (and, for L > 64, in wave
+// order through red).  Wing lines then add to the nodes at the record's edges.  Ends with
+// a barrier.  red: 4 x 6 doubles of LDS.
 template <typename R>
 __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds<R> &lsrc, bool staged, int t,
-                                       double *g) {
+                                       double *g, double *red) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int nd = wv; nd < WAVE_NODES; nd += BLOCK / 64) {
-        int k = 1;
-        while (k < 4 && nd >= lvl_node_off(k + 1)) k++;
-        const int qloc = nd - lvl_node_off(k), span = 2 << k;
-        const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-        Acc aL, aP;
-        aL.zero();
-        aP.zero();
-        gather_level_node<R>(gsrc, k, t * span + qloc, nL, nP, lane, 64, aL, aP);
-        double vL = aL.value(), vP = aP.value();
-        for (int off = 32; off > 0; off >>= 1) {
-            vL += __shfl_xor(vL, off);
-            vP += __shfl_xor(vP, off);
+#pragma unroll 1
+    for (int k = 1; k <= 9; k++) {
+        const int npair = 1 << (k - 1), L = BLOCK >> (k - 1);
+        const int pi = tid / L, sl = tid % L;
+        const int gl = t * npair + 1 + pi;         // left-walk group of the pair
+        const int gr = (5 << k) + 1 - gl;          // right-walk group (NT_k / 2 + 1 - gl)
+        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (k >= 4 && staged) {
+            pair_walk<R, false>(lsrc, k, gl, sl, L, s);
+            pair_walk<R, true>(lsrc, k, gr, sl, L, s);
+        } else {
+            pair_walk<R, false>(gsrc, k, gl, sl, L, s);
+            pair_walk<R, true>(gsrc, k, gr, sl, L, s);
         }
-        if (lane == 0) store_level_node(g, k, t, qloc, vL, vP);
+        for (int off = (L < 64 ? L : 64) >> 1; off > 0; off >>= 1)
+#pragma unroll
+            for (int c = 0; c < 6; c++) s[c] += __shfl_xor(s[c], off);
+        const int nk = 1 << k, accb = lvl_acc_off(k);
+        if (L > 64) {  // pairs spanning several waves: wave partials in wave order
+            if (lane == 0)
+#pragma unroll
+                for (int c = 0; c < 6; c++) red[wv * 6 + c] = s[c];
+            __syncthreads();
+            if (tid < npair) {
+                const int w0 = tid * (L >> 6), w1 = w0 + (L >> 6);
+#pragma unroll
+                for (int c = 0; c < 6; c++) {
+                    double x = red[w0 * 6 + c];
+                    for (int w = w0 + 1; w < w1; w++) x += red[w * 6 + c];
+                    s[c] = x;
+                }
+            }
+            __syncthreads();
+        }
+        if (L > 64 ? tid < npair : sl == 0) {
+            const int c1 = 2 * (L > 64 ? tid : pi);  // tile cells 2pi+1, 2pi+2 (0-based 2pi, 2pi+1)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                g[accb + c1 + c] = s[3 * c];
+                g[accb + nk + c1 + c] = s[3 * c + 1];
+                g[accb + 2 * nk + c1 + c] = s[3 * c + 2];
+            }
+        }
     }
-    for (int nd = WAVE_NODES + tid; nd < NLVL_NODES; nd += BLOCK) {
-        int k = 5;
-        while (k < 9 && nd >= lvl_node_off(k + 1)) k++;
-        const int qloc = nd - lvl_node_off(k), span = 2 << k, q = t * span + qloc;
-        const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-        Acc aL, aP;
-        aL.zero();
-        aP.zero();
-        if (staged) gather_level_centre<R>(lsrc, k, q, nL, nP, 0, 1, aL, aP);
-        else gather_level_centre<R>(gsrc, k, q, nL, nP, 0, 1, aL, aP);
-        gather_level_wings<R>(gsrc, k, q, nL, nP, 0, 1, aL, aP);
-        store_level_node(g, k, t, qloc, aL.value(), aP.value());
+    if (t == 0 || t == TILES - 1) {
+        // wing lines within H_{k-1} of the record's edge: the five position nodes of each grid
+        // at that edge (tile 0: qloc 0..4; last tile: span-4 .. span).  Grids 1..4 one wave per
+        // node (lanes split the lines), grids 5..9 one thread per node.
+        __syncthreads();
+        for (int nd = wv; nd < 20; nd += BLOCK / 64) {
+            const int k = 1 + nd / 5, e = nd % 5, span = 2 << k;
+            const int qloc = t == 0 ? e : span - e;
+            const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
+            Acc aL, aP;
+            aL.zero();
+            aP.zero();
+            gather_level_wings<R>(gsrc, k, t * span + qloc, nL, nP, lane, 64, aL, aP);
+            double vL = aL.value(), vP = aP.value();
+            for (int off = 32; off > 0; off >>= 1) {
+                vL += __shfl_xor(vL, off);
+                vP += __shfl_xor(vP, off);
+            }
+            if (lane == 0) add_level_node(g, k, qloc, vL, vP);
+        }
+        if (tid < 25) {
+            const int k = 5 + tid / 5, e = tid % 5, span = 2 << k;
+            const int qloc = t == 0 ? e : span - e;
+            const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
+            Acc aL, aP;
+            aL.zero();
+            aP.zero();
+            gather_level_wings<R>(gsrc, k, t * span + qloc, nL, nP, 0, 1, aL, aP);
+            add_level_node(g, k, qloc, aL.value(), aP.value());
+        }
     }
     __syncthreads();
 }
@@ -496,7 +533,7 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t) {
 // One (record, tile) work item w by node-owner gathers: grid 1..9 sums into g, cascade,
 // fine points in registers, record write.  LDS: g[NACC], staging arrays of capacity cap.
 template <typename R>
-__device__ inline void fine_tile(const KArgs &a, int w, double *g, double *s_u, int32_t *s_j, int32_t *s_b,
+__device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, double *s_u, int32_t *s_j, int32_t *s_b,
                                  int *s_cnt, int cap) {
     const int r = w / TILES, t = w % TILES;
     const int lev = r / a.nrec, s = r % a.nrec;
@@ -507,7 +544,7 @@ __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *s_u, 
     const GlobalSrc<R> gsrc(f);
     CentreLds<R> lsrc;
     const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc, cap);
-    tile_level_sums<R>(gsrc, lsrc, staged, t, g);
+    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red);
 
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
     if (a.grids && r == 0) {
@@ -573,10 +610,11 @@ __global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
     __shared__ int32_t s_j[CAP];
     __shared__ int32_t s_b[STAGE_NB + 1];
     __shared__ int s_cnt[BLOCK / 64];
+    __shared__ double red[BLOCK / 64 * 6];  // wave partials of the level sums
     const int nitems = a.nlev * a.nrec * TILES;
     const int w = XCD ? xcd_item(nitems) : (int)blockIdx.x;
     if (w >= nitems) return;
-    fine_tile<R>(a, w, g, s_u, s_j, s_b, s_cnt, CAP);
+    fine_tile<R>(a, w, g, red, s_u, s_j, s_b, s_cnt, CAP);
 }
 
 // ---- adjoint of the deposit + cascade of one (record, tile)
@@ -593,6 +631,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     __shared__ int32_t s_j[STAGE_CAP];
     __shared__ int32_t s_b[STAGE_NB + 1];
     __shared__ int s_cnt[BLOCK / 64];
+    __shared__ double red[BLOCK / 64 * 6];
     const int nitems = a.nlev * a.nrec * TILES;
     const int w = xcd_item(nitems);
     if (w >= nitems) return;
@@ -605,7 +644,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     const GlobalSrc<R> gsrc(f);
     CentreLds<R> lsrc;
     const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
-    tile_level_sums<R>(gsrc, lsrc, staged, t, g);
+    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red);
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
     tile_cascade(g, cz, t);
 
