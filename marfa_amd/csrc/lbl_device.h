@@ -108,6 +108,10 @@ struct LineShape {
     // region decisions in FP64: VX = |X| xs64 (within 2 ulp of the FP64 form's
     // |sqrt(ln2) X / gD|), VY = vy64 formed as the FP64 form forms it
     double xs64, vy64;
+    // in the level's sorted arrays: shifted centre and line index, so that one load gives a
+    // gather everything it tests and evaluates
+    double nus;
+    int32_t idx;
 };
 
 template <typename R>

@@ -33,14 +33,19 @@ __device__ inline int bucket_of(double nus, double base, int NB) {
 // profile parameters of every line, in sorted order, for the fine gathers
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void shape_kernel(const int32_t *sidx, int64_t nl, int64_t lmin,
-                                                      const double *gD, const double *gL, const double *S,
-                                                      const double *sscale, int lev0, LineShape<R> *shp) {
+                                                      const double *nus, const double *gD, const double *gL,
+                                                      const double *S, const double *sscale, int lev0,
+                                                      LineShape<R> *shp) {
     const int lev = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nl) return;
     const int64_t o = (int64_t)lev * nl;
-    const int64_t i = o + (sidx[o + j] - lmin);
-    shp[o + j] = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
+    const int32_t li = sidx[o + j];
+    const int64_t i = o + (li - lmin);
+    LineShape<R> p = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
+    p.nus = nus[i];
+    p.idx = li;
+    shp[o + j] = p;
 }
 
 template <typename R>
@@ -90,11 +95,29 @@ template <typename R>
 struct GlobalSrc {
     const FineCtx<R> &f;
     const ShapeCtl &ctl;
-    static constexpr bool kPrefetch = false;  // most candidates are rejected before their shape is read
     __device__ GlobalSrc(const FineCtx<R> &f_) : f(f_), ctl(f_.ctl) {}
     __device__ inline void range(double u0, double u1, int &j0, int &j1) const { cand(f, u0, u1, j0, j1); }
     __device__ inline int cls(int j, double &u) const { return classify(f, j, u); }
     __device__ inline const LineShape<R> &shape(int j) const { return f.shp[j]; }
+    __device__ inline const LineShape<R> &line(int j) const { return f.shp[j]; }
+    // classify()'s class 2 from the sorted entry's own record
+    __device__ inline bool centre(const LineShape<R> &p, double &u) const {
+        u = p.nus - f.org;
+        if (p.idx < f.wlo || p.idx >= f.whi) return false;
+        if (p.nus < f.org || p.nus > f.top) return false;
+        return u < 10.0;
+    }
+    // candidates of a cell pair's group (u in [wlo, whi) on the left walk, v = 10 - u in
+    // [wlo, whi) on the right walk) and of a grid-9 cell's fine points (u in [ulo, uhi)):
+    // bucket supersets, the gathers test exactly
+    template <bool RIGHT>
+    __device__ inline void group_range(double wlo, double whi, int &j0, int &j1) const {
+        if (!RIGHT) cand(f, wlo, whi, j0, j1);
+        else cand(f, 10.0 - whi - 1e-9, 10.0 - wlo + 1e-9, j0, j1);
+    }
+    __device__ inline void fine_range(double ulo, double uhi, int &j0, int &j1) const {
+        cand(f, ulo - 1e-9, uhi, j0, j1);
+    }
 };
 
 // Centre lines (class 2) of one tile's fine reach, staged in LDS in sorted order: u = nu* - org
@@ -112,55 +135,39 @@ struct CentreLds {
     const double *su;
     const int32_t *sj;
     const int32_t *sb;
-    double ustart;
+    double ustart, org;
     const LineShape<R> *shp;
     ShapeCtl ctl;
-    // every staged line is a centre line; the FP64 form passes the shape to an out-of-line
-    // call, where a register copy would go through scratch
-    static constexpr bool kPrefetch = sizeof(R) == 4;
-    // the local buckets give a superset; the staged u are sorted, so two binary searches
-    // inside it give exactly the lines with u in [u0, u1)
-    __device__ inline void range(double u0, double u1, int &j0, int &j1) const {
-        const double x0 = (u0 - ustart) * BUCKETS_PER_CM, x1 = (u1 - ustart) * BUCKETS_PER_CM;
-        const int b0 = (int)fmin(fmax(floor(x0) - 1.0, 0.0), (double)STAGE_NB);
-        const int b1 = (int)fmin(fmax(floor(x1) + 2.0, 0.0), (double)STAGE_NB);
-        const int a0 = sb[b0], a1 = b1 > b0 ? sb[b1] : a0;
-        j0 = lower(a0, a1, u0);
-        j1 = lower(j0, a1, u1);
+    // every staged line is a centre line
+    __device__ inline const LineShape<R> &line(int j) const { return shp[sj[j]]; }
+    __device__ inline bool centre(const LineShape<R> &p, double &u) const {
+        u = p.nus - org;  // the value staged in su
+        return true;
     }
-    // first staged index in [lo, hi) with u >= x (hi when none)
-    __device__ inline int lower(int lo, int hi, double x) const {
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (su[mid] < x) lo = mid + 1;
-            else hi = mid;
+    // staged bucket of a position on the 1/512 cm-1 grid (exact: ustart is on that grid)
+    __device__ inline int bk(double x) const { return (int)((x - ustart) * BUCKETS_PER_CM); }
+    // Group boundaries of grids 4..9 (multiples of H_{k-1} from the origin) and the fine
+    // cells' boundaries are on the staged bucket grid, so the bucket table gives the ranges
+    // without a search.  Left walk: exactly u in [wlo, whi).  Right walk: u >= 10 - whi is a
+    // superset at the start (the gather tests v); the end steps past the lines whose
+    // v = fl(10 - u) still rounds to >= wlo.
+    template <bool RIGHT>
+    __device__ inline void group_range(double wlo, double whi, int &j0, int &j1) const {
+        if (!RIGHT) {
+            j0 = sb[bk(wlo)];
+            j1 = sb[bk(whi)];
+        } else {
+            j0 = sb[bk(10.0 - whi)];
+            j1 = sb[bk(10.0 - wlo)];
+            const int n = sb[STAGE_NB];
+            while (j1 < n && !(10.0 - su[j1] < wlo)) j1++;
         }
-        return lo;
     }
-    __device__ inline int cls(int j, double &u) const {
-        u = su[j];
-        return 2;
+    __device__ inline void fine_range(double ulo, double uhi, int &j0, int &j1) const {
+        j0 = sb[bk(ulo)];
+        j1 = sb[bk(uhi)];
     }
-    __device__ inline const LineShape<R> &shape(int j) const { return shp[sj[j]]; }
 };
-
-// Visit candidates j = j0, j0 + step, ... < j1 with their line parameters.  For the staged
-// source (only centre lines) the next line's parameters are loaded before the current line
-// is evaluated, so that the global load overlaps the evaluation.
-template <typename R, typename Src, typename F>
-__device__ inline void for_lines(const Src &src, int j0, int j1, int step, F &&f) {
-    if constexpr (Src::kPrefetch) {
-        if (j0 >= j1) return;
-        LineShape<R> nxt = src.shape(j0);
-        for (int j = j0; j < j1; j += step) {
-            const LineShape<R> cur = nxt;
-            if (j + step < j1) nxt = src.shape(j + step);
-            f(j, cur);
-        }
-    } else {
-        for (int j = j0; j < j1; j += step) f(j, src.shape(j));
-    }
-}
 
 // Centre lines on grid k, by cell pairs.  The left walk of a centre line with
 // n = floor(u 2^k) >= 2 covers cells 2g-1 .. n-1 of grid k, g = n >> 1 (positions
@@ -177,31 +184,37 @@ __device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, do
     const double wlo = (double)(2 * g) * Hk, whi = (double)(2 * g + 2) * Hk;
     const double q0 = (double)(4 * g - 4);
     int j0, j1;
-    if (!RIGHT) src.range(wlo, whi, j0, j1);
-    else src.range(10.0 - whi - 1e-9, 10.0 - wlo + 1e-9, j0, j1);
+    src.template group_range<RIGHT>(wlo, whi, j0, j1);
     // sums by position e = q - q0 (a2o: e = 2 of lines with n odd, the second cell's edge)
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a2o = 0.0, a3 = 0.0, a4 = 0.0;
-    for_lines<R>(src, j0 + sl, j1, L, [&](int j, const LineShape<R> &p) {
-        double u;
-        if (src.cls(j, u) != 2) return;
-        const double w = RIGHT ? 10.0 - u : u;  // distance walked from the pair's side
-        if (w < wlo || !(w < whi)) return;
-        const bool odd = (int)floor(__builtin_ldexp(w, k)) & 1;
-        const int ne = odd ? 5 : 3;
-        // X = w - q hq at q = q0 + e (the reference's positions: left walk u - x_q, right
-        // walk v - x_q' in mirrored positions)
+    int j = j0 + sl;
+    if (j < j1) {
+        // the next line's record is loaded while the current one is evaluated
+        LineShape<R> nxt = src.line(j);
+        for (; j < j1; j += L) {
+            const LineShape<R> p = nxt;
+            if (j + L < j1) nxt = src.line(j + L);
+            double u;
+            if (!src.centre(p, u)) continue;
+            const double w = RIGHT ? 10.0 - u : u;  // distance walked from the pair's side
+            if (w < wlo || !(w < whi)) continue;
+            const bool odd = (int)floor(__builtin_ldexp(w, k)) & 1;
+            const int ne = odd ? 5 : 3;
+            // X = w - q hq at q = q0 + e (the reference's positions: left walk u - x_q, right
+            // walk v - x_q' in mirrored positions)
 #pragma unroll 1
-        for (int e = 0; e < ne; e++) {
-            const double v = (double)shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
-            if (e == 0) a0 += v;
-            else if (e == 1) a1 += v;
-            else if (e == 2) {
-                a2 += v;
-                if (odd) a2o += v;
-            } else if (e == 3) a3 += v;
-            else a4 += v;
+            for (int e = 0; e < ne; e++) {
+                const double v = (double)shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
+                if (e == 0) a0 += v;
+                else if (e == 1) a1 += v;
+                else if (e == 2) {
+                    a2 += v;
+                    if (odd) a2o += v;
+                } else if (e == 3) a3 += v;
+                else a4 += v;
+            }
         }
-    });
+    }
     // left walk: e -> P1, M1, L1 (+P2 for n odd), M2, L2; right walk: the mirror image
     if (!RIGHT) {
         s[0] += a0; s[1] += a1; s[2] += a2; s[3] += a2o; s[4] += a3; s[5] += a4;
@@ -267,25 +280,42 @@ __device__ inline void gather_level_wings(const Src &src, int k, int q, bool nee
     }
 }
 
-// direct deposits at fine point i (1-based record index, position (i-1) H): centre lines
+// direct deposits of centre lines at the four fine points i0 .. i0+3 of one grid-9 cell
+// (i0 = 4m + 2, positions (i-1) H).  Point i takes the line at or right of it when
+// u < (floor((i-2)/4) + 2) H9 and the line left of it when v = 10 - u <
+// (floor((NT-i)/4) + 2) H9 (src/LineGridCalc.f90:332-339, :485-490); both bounds are the
+// same for the four points of a cell, so they share one candidate range, u in
+// [(m-1) H9, (m+2) H9), and each line is loaded once.  v[e]: sums in sorted order.
 template <typename R, typename Src>
-__device__ inline void gather_fine_centre(const Src &src, int i, Acc &acc) {
-    const double xi = (double)(i - 1) * HF;
-    {   // centre lines whose core reaches point i.  Left part (the line at or right of the
-        // point): u in [(i-1)H, (floor((i-2)/4) + 2) H9).  Right part (the line left of the
-        // point): u < (i-1)H and v = 10 - u < (floor((NT-i)/4) + 2) H9.  Both sides are
-        // contiguous in the sorted order, so one candidate range serves them.
-        const double uhi = (double)(((i - 2) >> 2) + 2) * H9;
-        const double vhi = (double)(((NT - i) >> 2) + 2) * H9;
-        int j0, j1;
-        src.range(10.0 - vhi - 1e-9, uhi, j0, j1);
-        for_lines<R>(src, j0, j1, 1, [&](int j, const LineShape<R> &p) {
+__device__ inline void gather_fine_cell(const Src &src, int i0, double v[4]) {
+    const double uhi = (double)(((i0 - 2) >> 2) + 2) * H9;
+    const double vhi = (double)(((NT - i0) >> 2) + 2) * H9;
+    int j0, j1;
+    src.fine_range(10.0 - vhi, uhi, j0, j1);
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (j0 < j1) {
+        LineShape<R> nxt = src.line(j0);
+        for (int j = j0; j < j1; j++) {
+            const LineShape<R> p = nxt;
+            if (j + 1 < j1) nxt = src.line(j + 1);
             double u;
-            if (src.cls(j, u) != 2) return;
-            if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) return;
-            acc.add(shape_eval<R, true>(u - xi, p, src.ctl));
-        });
+            if (!src.centre(p, u)) continue;
+#pragma unroll 1
+            for (int e = 0; e < 4; e++) {
+                const double xi = (double)(i0 + e - 1) * HF;
+                if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) continue;
+                const double x = (double)shape_eval<R, true>(u - xi, p, src.ctl);
+                if (e == 0) v0 += x;
+                else if (e == 1) v1 += x;
+                else if (e == 2) v2 += x;
+                else v3 += x;
+            }
+        }
     }
+    v[0] = v0;
+    v[1] = v1;
+    v[2] = v2;
+    v[3] = v3;
 }
 
 // direct deposits of wing lines at the first / last fine points of the record
@@ -396,6 +426,7 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
     lsrc.sj = s_j;
     lsrc.sb = s_b;
     lsrc.ustart = ustart;
+    lsrc.org = f.org;
     lsrc.shp = f.shp;
     lsrc.ctl = f.ctl;
     return staged;
@@ -566,23 +597,27 @@ __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, 
     const int64_t ro = ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
     for (int h = 0; h < 2; h++) {
         const int c = tid + h * BLOCK;
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        const int i0 = 2048 * t + 2 + 4 * c;
+        double v[4];
+        if (staged) gather_fine_cell<R>(lsrc, i0, v);
+        else gather_fine_cell<R>(gsrc, i0, v);
+        if (i0 <= 5 || i0 + 3 >= NT - 3) {
 #pragma unroll 1
-        for (int e = 0; e < 4; e++) {
-            const int il = 4 * c + e, i = 2048 * t + 2 + il;
-            Acc acc;
-            acc.zero();
-            if (staged) gather_fine_centre<R>(lsrc, i, acc);
-            else gather_fine_centre<R>(gsrc, i, acc);
-            if (i <= 5 || i >= NT - 3) gather_fine_wings<R>(gsrc, i, acc);
-            const double x = acc.value();
-            // register-resident selection (no dynamic indexing of v)
-            v[0] = e == 0 ? x : v[0];
-            v[1] = e == 1 ? x : v[1];
-            v[2] = e == 2 ? x : v[2];
-            v[3] = e == 3 ? x : v[3];
-            if (a.grids && r == 0) a.grids[30690 + 2048 * t + 1 + il] = x / sc;
+            for (int e = 0; e < 4; e++) {
+                const int i = i0 + e;
+                if (i > 5 && i < NT - 3) continue;
+                Acc acc;
+                acc.zero();
+                gather_fine_wings<R>(gsrc, i, acc);
+                const double x = acc.value();
+                v[0] += e == 0 ? x : 0.0;
+                v[1] += e == 1 ? x : 0.0;
+                v[2] += e == 2 ? x : 0.0;
+                v[3] += e == 3 ? x : 0.0;
+            }
         }
+        if (a.grids && r == 0)
+            for (int e = 0; e < 4; e++) a.grids[30690 + 2048 * t + 1 + 4 * c + e] = v[e] / sc;
         const double P = g[cb9 + c], M = g[cb9 + 512 + c], L = g[cb9 + 1024 + c];
         bool l1, l2;
         v[0] += cascade_a1(P, M, L, l1);
@@ -603,8 +638,10 @@ __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, 
 
 // CAP: centre lines staged in LDS per tile (LDS 27.5 KB + 12 B x CAP: occupancy against
 // staging); XCD: contiguous record ranges per XCD (xcd_item) or plain dispatch order.
+// The FP32 form is held to 128 VGPRs (4 waves per SIMD, as many as the LDS of the default
+// stage allows): the gathers wait on memory, and waves hide that better than registers.
 template <typename R, int CAP = STAGE_CAP, bool XCD = true>
-__global__ __launch_bounds__(BLOCK) void fine_kernel(KArgs a) {
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) void fine_kernel(KArgs a) {
     __shared__ double g[NACC];            // grids 1..9: P, mid, L of the tile's cells
     __shared__ double s_u[CAP];           // staged centre lines
     __shared__ int32_t s_j[CAP];

@@ -651,11 +651,11 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             dim3 gsh((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
             if (job.precision == MARFA_FP32K)
                 hipLaunchKernelGGL(shape_kernel<float>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->s->idx[1].p, nl,
-                                   lmin, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
+                                   lmin, (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
                                    (const double *)dsc, l0, (LineShape<float> *)c->s->shapes.p);
             else
                 hipLaunchKernelGGL(shape_kernel<double>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->s->idx[1].p, nl,
-                                   lmin, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
+                                   lmin, (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
                                    (const double *)dsc, l0, (LineShape<double> *)c->s->shapes.p);
             HIPCHK(hipGetLastError());
         }
