@@ -21,7 +21,6 @@
 // m = #{k : d < H_k}, and into fine points 2..5 (NT-3..NT) when m = 10.
 #pragma once
 
-constexpr double BUCKETS_PER_CM = 512.0;
 
 __device__ inline int bucket_of(double nus, double base, int NB) {
     const double x = (nus - base) * BUCKETS_PER_CM;

@@ -80,6 +80,9 @@ struct KArgs {
                     // (grids 0..9 as P|mid|L blocks, then RK(1..20481)), sample scale removed
 };
 
+// Buckets of the per-level index of sorted shifted centres (1/512 cm-1 wide).
+constexpr double BUCKETS_PER_CM = 512.0;
+
 // Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a contiguous
 // range of work items so that neighbouring records (which share 90+% of their line
 // windows) are processed under the same L2.  The grid is padded to 8*per blocks.
@@ -209,10 +212,45 @@ __device__ inline bool level0_masks(double nus, double org, double cut, uint32_t
     return true;
 }
 
-// FP32 form: registers capped for 3 waves per SIMD (fits without spills); the FP64 form's
-// larger state would spill under that cap and runs at 2
+// chi bands of chi_factor_sel for Tonkov (CHI 1) and Pollack (CHI 2): band index of |X| and
+// the band's law coef * exp(-(rate |X|)) (band 0: chi = 1)
 template <typename R, int CHI>
-__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 1) void coarse_kernel(KArgs a) {
+__device__ inline int chi_band(R ax) {
+    if constexpr (CHI == 1) return ax > R(3) ? (ax <= R(150) ? 1 : ax <= R(300) ? 2 : 3) : 0;
+    else return ax >= R(3) ? (ax < R(10) ? 1 : 2) : 0;
+}
+template <typename R, int CHI>
+__device__ inline void chi_law(int b, R &coef, R &rate) {
+    if constexpr (CHI == 1) {
+        coef = b == 0 ? R(1) : b == 1 ? R(1.084f) : b == 2 ? R(0.208f) : R(0.025f);
+        rate = b == 0 ? R(0) : b == 1 ? R(0.027f) : b == 2 ? R(0.016f) : R(0.009f);
+    } else {
+        coef = b == 0 ? R(1) : b == 1 ? R(1.35f) : R(0.614f);
+        rate = b == 0 ? R(0) : b == 1 ? R(0.1) : R(1.0 / 47.0);
+    }
+}
+
+// One workgroup per record.  Threads take the level's sorted line records (shifted centre,
+// line index and the precomputed profile parameters) in order, so the 64 lines of a
+// wavefront lie within a fraction of a cm-1 and take the same branches.  A line at least
+// 1 cm-1 outside the subinterval whose nearest node is on the Lorentz branch deposits
+// into all 21 nodes at |X| = d + q/2 (left wing) or d + (20-q)/2 (right wing), taken in
+// precision R from d >= 1 (relative error ~1 ulp).  When chi keeps one band over those
+// nodes (Tonkov / Pollack), chi(|X| +- 1/2) = chi(|X|) exp(-+rate/2): chi is evaluated at
+// the first node and carried to the others by that FP64 ratio, one exp per line instead of
+// one per sample.  Every other line takes the masked per-node path with X = u - q/2.
+// Both forms run at 2 waves per SIMD: the 31 FP64 node sums and the three unrolled node
+// loops do not fit the register budget of 3 without spilling the sums.
+template <typename R, int CHI>
+__global__ __launch_bounds__(BLOCK, 2) void coarse_kernel(KArgs a) {
+    constexpr bool kBands = CHI == 1 || CHI == 2;
+    __shared__ double rho_tab[8];  // exp(-rate_b * step), step = +1/2 (even entries), -1/2
+    if (kBands && threadIdx.x < 8) {
+        R coef, rate;
+        chi_law<R, CHI>(threadIdx.x >> 1, coef, rate);
+        rho_tab[threadIdx.x] = exp(-(double)rate * ((threadIdx.x & 1) ? -0.5 : 0.5));
+    }
+    __syncthreads();
     const int nitems = a.nlev * a.nrec;
     const int r = xcd_item(nitems);
     if (r >= nitems) return;
@@ -220,23 +258,43 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 1) void coarse_kernel(K
     const double org = a.org[s];
     const int64_t lo = a.lo[s], hi = a.hi[s];
     const double cut = (double)a.cutoff;
-    const double top = org + 10.0;  // endDeltaWV
     const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
-    const double sc = a.sscale[a.lev0 + lev];
+    const bool nochi = CHI == 0 || ctl.mol != 2;
+    const LineShape<R> *shp = reinterpret_cast<const LineShape<R> *>(a.shp) + (int64_t)lev * a.nl;
+    const int32_t *bst = a.bst + (int64_t)lev * (a.NB + 1);
+    // sorted lines with shifted centres in [org - cut, org + 10 + cut] (bucket superset; the
+    // window and deposit tests below are exact)
+    int J0, J1;
+    {
+        const double x0 = (org - cut - a.bbase) * BUCKETS_PER_CM, x1 = (org + 10.0 + cut - a.bbase) * BUCKETS_PER_CM;
+        const int b0 = (int)fmin(fmax(floor(x0) - 1.0, 0.0), (double)a.NB);
+        const int b1 = (int)fmin(fmax(floor(x1) + 2.0, 0.0), (double)a.NB);
+        J0 = bst[b0];
+        J1 = b1 > b0 ? bst[b1] : J0;
+    }
 
     Acc aR, aP[10], aM[10], aL[10];
     aR.zero();
 #pragma unroll
     for (int c = 0; c < 10; c++) { aP[c].zero(); aM[c].zero(); aL[c].zero(); }
+    // node q: position q/2 of the subinterval; even q is L of cell q/2 and P of cell q/2+1
+    auto add_all = [&](int q, double val) {
+        if (q == 0) aR.add(val);
+        if (q & 1) {
+            aM[q >> 1].add(val);
+        } else {
+            if (q <= 18) aP[q >> 1].add(val);
+            if (q >= 2) aL[(q >> 1) - 1].add(val);
+        }
+    };
 
-    const int64_t base = (int64_t)lev * a.nl - a.lmin;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) {
-        const double nus = a.nus[base + i];
+    for (int j = J0 + threadIdx.x; j < J1; j += BLOCK) {
+        const LineShape<R> p = shp[j];
+        if (p.idx < lo || p.idx >= hi) continue;  // the record's window (unshifted centres)
+        const double nus = p.nus;
         const double u = nus - org;
         uint32_t mP, mM, mL;
-        const bool use = level0_masks(nus, org, cut, mP, mM, mL);
-        if (!use) continue;
-        const LineShape<R> p = make_line<R>(a.gD[base + i], a.gL[base + i], a.S[base + i] * sc);
+        if (!level0_masks(nus, org, cut, mP, mM, mL)) continue;
         // a line at least 1 cm-1 outside the subinterval deposits into every level-0 node
         // (full masks); when even its nearest node, |X| >= dmin, is on the Lorentz branch
         // (|X| xs >= margin as shape_eval tests it; rounding is monotone, so every node is),
@@ -244,16 +302,37 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 1) void coarse_kernel(K
         const double dmin = -u >= 1.0 ? -u : u - 10.0;
         const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
         if (dmin >= 1.0 && (ctl.shape_id == 1 || (R)dmin * p.xs >= margin)) {
+            const bool lw = -u >= 1.0;
+            const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
+            const R step = lw ? R(0.5) : R(-0.5);
+            if (nochi) {
+#pragma unroll
+                for (int q = 0; q <= 20; q++) {
+                    const R ax = q == 0 ? base : fma(step, (R)q, base);
+                    add_all(q, (double)(p.lorc * recip<R>(ax * ax + p.gl2)));
+                }
+                continue;
+            }
+            if constexpr (kBands) {
+                const int b = chi_band<R, CHI>((R)dmin);
+                if (b == chi_band<R, CHI>((R)(dmin + 10.0))) {
+                    R coef, rate;
+                    chi_law<R, CHI>(b, coef, rate);
+                    double chi = (double)(coef * dexp<R>(-(rate * base)));
+                    const double rho = rho_tab[2 * b + (lw ? 0 : 1)];
+#pragma unroll
+                    for (int q = 0; q <= 20; q++) {
+                        const R ax = q == 0 ? base : fma(step, (R)q, base);
+                        if (q > 0) chi *= rho;
+                        add_all(q, (double)(p.lorc * recip<R>(ax * ax + p.gl2)) * chi);
+                    }
+                    continue;
+                }
+            }
 #pragma unroll
             for (int q = 0; q <= 20; q++) {
-                const R val = chilor_abs<R, CHI>((R)fabs(u - 0.5 * q), p, ctl);
-                if (q == 0) aR.add(val);
-                if (q & 1) {
-                    aM[q >> 1].add(val);
-                } else {
-                    if (q <= 18) aP[q >> 1].add(val);
-                    if (q >= 2) aL[(q >> 1) - 1].add(val);
-                }
+                const R ax = q == 0 ? base : fma(step, (R)q, base);
+                add_all(q, (double)chilor_abs<R, CHI>(ax, p, ctl));
             }
             continue;
         }
@@ -262,13 +341,13 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 1) void coarse_kernel(K
         for (int q = 0; q <= 20; q++) {
             // positions no role needs are moved far out, onto the cheap Lorentz branch
             const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
-            const R val = shape_eval<R, false, CHI>(X, p, ctl);
+            const double val = (double)shape_eval<R, false, CHI>(X, p, ctl);
             if (q == 0) aR.add(val);
             if (q & 1) {
-                aM[q >> 1].add(((mM >> q) & 1u) ? val : R(0));
+                aM[q >> 1].add(((mM >> q) & 1u) ? val : 0.0);
             } else {
-                if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : R(0));
-                if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : R(0));
+                if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : 0.0);
+                if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : 0.0);
             }
         }
     }
@@ -299,7 +378,6 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 1) void coarse_kernel(K
 }
 
 // ------------------------------------------------------------------ 4. fine + cascade
-constexpr int NLVL_NODES = 2053;  // sum_{k=1..9} (2^(k+1)+1) position nodes per tile
 #include "lbl_fine.h"
 #include "lbl_adjoint.h"
 #include "lbl_sort.h"
