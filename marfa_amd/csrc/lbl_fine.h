@@ -106,6 +106,18 @@ struct GlobalSrc {
         if (p.nus < f.org || p.nus > f.top) return false;
         return u < 10.0;
     }
+    // classify()'s classes 1 and 3 (wing lines closer than 1 cm-1 to the record's left /
+    // right edge), from the sorted entry's own record
+    __device__ inline bool wing_left(const LineShape<R> &p, double u) const {
+        if (p.idx < f.wlo || p.idx >= f.whi || !(p.nus < f.org)) return false;
+        const double d = -u;
+        return !(d > f.cut) && d < 1.0;
+    }
+    __device__ inline bool wing_right(const LineShape<R> &p, double u) const {
+        if (p.idx < f.wlo || p.idx >= f.whi || !(p.nus > f.top)) return false;
+        const double dr = u - 10.0;
+        return !(dr >= f.cut) && dr < 1.0;
+    }
     // candidates of a cell pair's group (u in [wlo, whi) on the left walk, v = 10 - u in
     // [wlo, whi) on the right walk) and of a grid-9 cell's fine points (u in [ulo, uhi)):
     // bucket supersets, the gathers test exactly
@@ -143,6 +155,9 @@ struct CentreLds {
         u = p.nus - org;  // the value staged in su
         return true;
     }
+    // wing lines are not staged (the record-edge pairs read global memory)
+    __device__ inline bool wing_left(const LineShape<R> &, double) const { return false; }
+    __device__ inline bool wing_right(const LineShape<R> &, double) const { return false; }
     // staged bucket of a position on the 1/512 cm-1 grid (exact: ustart is on that grid)
     __device__ inline int bk(double x) const { return (int)((x - ustart) * BUCKETS_PER_CM); }
     // Group boundaries of grids 4..9 (multiples of H_{k-1} from the origin) and the fine
@@ -177,11 +192,18 @@ struct CentreLds {
 // visits each line of its two groups once and evaluates 3 positions (n even) or 5 (n odd).
 // The lines of a group are split over the owner's L threads (sub-lane sl), each in sorted
 // order.  Slots: 0..2 = P, mid, L of cell 2g-1; 3..5 = the same of cell 2g.
+// The record's edge pairs (cells 1, 2 and NT_k-1, NT_k) also take the wing lines closer
+// than H_{k-1} to that edge (leftLBL / rightLBL on grids k <= m, src/LineGridCalc.f90): they
+// lie in the u range of the edge pair's mirror group, and a wing line's positions follow
+// the same pattern as a centre line's -- nodes q = 4-e from the edge, e = 0..2, plus
+// e = 3, 4 and the second cell's edge when d >= H_k (m == k) -- so they are summed in the
+// same loop.  X: left wing u - q hq, right wing (u - 10) + q' hq, as the reference forms them.
 template <typename R, bool RIGHT, typename Src>
 __device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, double s[6]) {
     const double Hk = hk(k), hq = 0.5 * Hk;
     const double wlo = (double)(2 * g) * Hk, whi = (double)(2 * g + 2) * Hk;
     const double q0 = (double)(4 * g - 4);
+    const bool edge = g == (5 << k);  // walk group starting at the record's far edge
     int j0, j1;
     src.template group_range<RIGHT>(wlo, whi, j0, j1);
     // sums by position e = q - q0 (a2o: e = 2 of lines with n odd, the second cell's edge)
@@ -194,21 +216,39 @@ __device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, do
             const LineShape<R> p = nxt;
             if (j + L < j1) nxt = src.line(j + L);
             double u;
-            if (!src.centre(p, u)) continue;
-            const double w = RIGHT ? 10.0 - u : u;  // distance walked from the pair's side
-            if (w < wlo || !(w < whi)) continue;
-            const bool odd = (int)floor(__builtin_ldexp(w, k)) & 1;
-            const int ne = odd ? 5 : 3;
-            // X = w - q hq at q = q0 + e (the reference's positions: left walk u - x_q, right
-            // walk v - x_q' in mirrored positions)
+            // X_e = B - (C + e) hq (centre lines, wing on the right edge) or B + (C + e) hq
+            double B, C;
+            bool five, plus = false;
+            if (src.centre(p, u)) {
+                const double w = RIGHT ? 10.0 - u : u;  // distance walked from the pair's side
+                if (w < wlo || !(w < whi)) continue;
+                five = (int)floor(__builtin_ldexp(w, k)) & 1;
+                B = w;
+                C = q0;
+            } else {
+                if (!edge) continue;
+                if (RIGHT) {  // left wing, u in (-H_{k-1}, 0): X = u - (4 - e) hq
+                    if (!src.wing_left(p, u) || !(-u < 2.0 * Hk)) continue;
+                    five = -u >= Hk;
+                    B = u;
+                    plus = true;
+                } else {      // right wing: X = (u - 10) + (4 - e) hq
+                    if (!src.wing_right(p, u) || !(u - 10.0 < 2.0 * Hk)) continue;
+                    five = u - 10.0 >= Hk;
+                    B = u - 10.0;
+                }
+                C = -4.0;
+            }
+            const int ne = five ? 5 : 3;
 #pragma unroll 1
             for (int e = 0; e < ne; e++) {
-                const double v = (double)shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
+                const double m = (C + (double)e) * hq;
+                const double v = (double)shape_eval<R, true>(plus ? B + m : B - m, p, src.ctl);
                 if (e == 0) a0 += v;
                 else if (e == 1) a1 += v;
                 else if (e == 2) {
                     a2 += v;
-                    if (odd) a2o += v;
+                    if (five) a2o += v;
                 } else if (e == 3) a3 += v;
                 else a4 += v;
             }
@@ -219,63 +259,6 @@ __device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, do
         s[0] += a0; s[1] += a1; s[2] += a2; s[3] += a2o; s[4] += a3; s[5] += a4;
     } else {
         s[5] += a0; s[4] += a1; s[3] += a2; s[2] += a2o; s[1] += a3; s[0] += a4;
-    }
-}
-
-template <typename R, typename Src>
-__device__ inline void gather_level_wings(const Src &src, int k, int q, bool needL, bool needP,
-                                          int jfirst, int jstep, Acc &aL, Acc &aP) {
-    const int NTk = 10 << k;
-    const double Hk = hk(k), hq = 0.5 * Hk, xq = q * hq;
-    // wing lines closer than H_{k-1} to the left edge: cells 1 (m == k) and 2 (m >= k)
-    if (q <= 4) {
-        const double hkm1 = hk(k - 1);
-        int j0, j1;
-        src.range(-hkm1, 0.0, j0, j1);
-        for (int j = j0 + jfirst; j < j1; j += jstep) {
-            double u;
-            if (src.cls(j, u) != 1) continue;
-            const double d = -u;
-            if (!(d < hkm1)) continue;
-            const bool eq = d >= Hk;
-            bool toL, toP;
-            if (q == 0) { toL = false; toP = eq; }
-            else if (q == 1) { toL = eq; toP = false; }
-            else if (q == 2) { toL = eq; toP = true; }
-            else { toL = true; toP = false; }
-            toL = toL && needL;
-            toP = toP && needP;
-            if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(u - xq, src.shape(j), src.ctl);
-            if (toL) aL.add(val);
-            if (toP) aP.add(val);
-        }
-    }
-    // wing lines closer than H_{k-1} to the right edge: cells NT_k (m == k), NT_k - 1
-    if (2 * NTk - q <= 4) {
-        const int qp = 2 * NTk - q;
-        const double hkm1 = hk(k - 1);
-        int j0, j1;
-        src.range(10.0 - 1e-9, 10.0 + hkm1 + 1e-9, j0, j1);
-        for (int j = j0 + jfirst; j < j1; j += jstep) {
-            double u;
-            if (src.cls(j, u) != 3) continue;
-            const double dr = u - 10.0;
-            if (!(dr < hkm1)) continue;
-            const bool eq = dr >= Hk;
-            bool toL, toP;
-            if (qp == 0) { toL = eq; toP = false; }
-            else if (qp == 1) { toL = eq; toP = false; }
-            else if (qp == 2) { toL = true; toP = eq; }
-            else if (qp == 3) { toL = true; toP = false; }
-            else { toL = false; toP = true; }
-            toL = toL && needL;
-            toP = toP && needP;
-            if (!toL && !toP) continue;
-            const R val = shape_eval<R, true>(dr + qp * hq, src.shape(j), src.ctl);
-            if (toL) aL.add(val);
-            if (toP) aP.add(val);
-        }
     }
 }
 
@@ -346,18 +329,6 @@ __device__ inline void gather_fine_wings(const Src &src, int i, Acc &acc) {
 
 // offset of grid k's P block in the tile's node sums; mid at +2^k, L at +2*2^k
 __device__ inline int lvl_acc_off(int k) { return 3 * ((1 << k) - 2); }
-
-// adds a node's wing sums: even qloc is the L edge of tile cell qloc/2 and the P edge of
-// cell qloc/2 + 1, odd qloc the midpoint of cell (qloc+1)/2 (1-based tile cells)
-__device__ inline void add_level_node(double *g, int k, int qloc, double vL, double vP) {
-    const int nk = 1 << k, span = 2 << k, accb = lvl_acc_off(k);
-    if (qloc & 1) {
-        g[accb + nk + ((qloc + 1) >> 1) - 1] += vL;
-    } else {
-        if (qloc > 0) g[accb + 2 * nk + (qloc >> 1) - 1] += vL;
-        if (qloc < span) g[accb + (qloc >> 1)] += vP;
-    }
-}
 
 // ---- stages of one (record, tile) work item, shared by the forward and adjoint kernels
 
@@ -437,8 +408,7 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
 // spread over all threads.  Grids 1..3 read their lines from the level's sorted arrays
 // (their groups reach up to 2 cm-1 beyond the tile), grids 4..9 from the staged copy.
 // The L partial sums of a pair are reduced in a fixed butterfly (and, for L > 64, in wave
-// order through red).  Wing lines then add to the nodes at the record's edges.  Ends with
-// a barrier.  red: 4 x 6 doubles of LDS.
+// order through red).  Ends with a barrier.  red: 4 x 6 doubles of LDS.
 template <typename R>
 __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds<R> &lsrc, bool staged, int t,
                                        double *g, double *red) {
@@ -450,7 +420,9 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
         const int gl = t * npair + 1 + pi;         // left-walk group of the pair
         const int gr = (5 << k) + 1 - gl;          // right-walk group (NT_k / 2 + 1 - gl)
         double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (k >= 4 && staged) {
+        // the record's edge pairs also take wing lines, which are not staged
+        const bool edge = (t == 0 && pi == 0) || (t == TILES - 1 && pi == npair - 1);
+        if (k >= 4 && staged && !edge) {
             pair_walk<R, false>(lsrc, k, gl, sl, L, s);
             pair_walk<R, true>(lsrc, k, gr, sl, L, s);
         } else {
@@ -485,37 +457,6 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
                 g[accb + nk + c1 + c] = s[3 * c + 1];
                 g[accb + 2 * nk + c1 + c] = s[3 * c + 2];
             }
-        }
-    }
-    if (t == 0 || t == TILES - 1) {
-        // wing lines within H_{k-1} of the record's edge: the five position nodes of each grid
-        // at that edge (tile 0: qloc 0..4; last tile: span-4 .. span).  Grids 1..4 one wave per
-        // node (lanes split the lines), grids 5..9 one thread per node.
-        __syncthreads();
-        for (int nd = wv; nd < 20; nd += BLOCK / 64) {
-            const int k = 1 + nd / 5, e = nd % 5, span = 2 << k;
-            const int qloc = t == 0 ? e : span - e;
-            const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-            Acc aL, aP;
-            aL.zero();
-            aP.zero();
-            gather_level_wings<R>(gsrc, k, t * span + qloc, nL, nP, lane, 64, aL, aP);
-            double vL = aL.value(), vP = aP.value();
-            for (int off = 32; off > 0; off >>= 1) {
-                vL += __shfl_xor(vL, off);
-                vP += __shfl_xor(vP, off);
-            }
-            if (lane == 0) add_level_node(g, k, qloc, vL, vP);
-        }
-        if (tid < 25) {
-            const int k = 5 + tid / 5, e = tid % 5, span = 2 << k;
-            const int qloc = t == 0 ? e : span - e;
-            const bool nL = (qloc & 1) || qloc > 0, nP = !(qloc & 1) && qloc < span;
-            Acc aL, aP;
-            aL.zero();
-            aP.zero();
-            gather_level_wings<R>(gsrc, k, t * span + qloc, nL, nP, 0, 1, aL, aP);
-            add_level_node(g, k, qloc, aL.value(), aP.value());
         }
     }
     __syncthreads();
