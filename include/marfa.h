@@ -57,6 +57,8 @@ extern "C" {
 
 /* error codes */
 #define MARFA_OK 0
+#define MARFA_E_SPECIES 2     /* unsupported molecule (app/processParFile.f90:128-131) */
+#define MARFA_E_PARREAD 3     /* .par field Fortran's read rejects (app/processParFile.f90:154-157) */
 #define MARFA_E_WINDOW 9      /* line index outside the line list (src/LBL.f90:50-53) */
 #define MARFA_E_ARG 20        /* invalid argument */
 #define MARFA_E_STATE 21      /* lines / TIPS / levels not set */
@@ -177,6 +179,20 @@ int marfa_count_samples(marfa_ctx *ctx, const marfa_job *job, int64_t samples[2]
  * selection against a CPU restatement. */
 int marfa_grid_sums(marfa_ctx *ctx, const marfa_job *job, int32_t level, int32_t rec,
                     double *grids, float *record);
+
+/* ---- line-list ingest (host only, no device needed) ----
+ * marfa_species_code <- getSpeciesCode (app/processParFile.f90:189-223): molecule name
+ *   (blanks trimmed) -> two-digit HITRAN code "01".."12" in code[3]; MARFA_E_SPECIES and
+ *   "00" for any other name.
+ * marfa_par_convert  <- the program app/processParFile.f90:32-185: HITRAN .par text ->
+ *   36-byte direct-access records (f8 nu; f4 S, gamma_air, gamma_self, E'', n_T; i4
+ *   jointMolIso; f4 delta) in db_path, record r = input line r (a line with an unknown
+ *   isotopologue leaves its record unwritten, a hole of zeros; no record follows the
+ *   last line, as in the reference's fixture test/testDB.02).  *nrec: records in the file.
+ *   MARFA_E_SPECIES (2) / MARFA_E_PARREAD (3) as the reference's stop 2 / stop 3;
+ *   MARFA_E_ARG when a file cannot be opened or written. */
+int marfa_species_code(const char *species, char code[3]);
+int marfa_par_convert(const char *species, const char *par_path, const char *db_path, int64_t *nrec);
 
 /* average duration [ms] of the last marfa_pt_tables call's kernels, measured with HIP
  * events on the stream they ran on: [0] spectroscopy, [1] sort, [2] coarse deposit,

@@ -12,7 +12,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmarfa.so")
-HIP_SRC = [os.path.join(HERE, "csrc", "marfa_lbl.hip")]
+HIP_SRC = [os.path.join(HERE, "csrc", "marfa_lbl.hip"), os.path.join(HERE, "csrc", "par_ingest.cpp")]
 HIP_DEPS = HIP_SRC + [os.path.join(HERE, "csrc", "lbl_device.h"), os.path.join(ROOT, "include", "marfa.h")]
 
 # packed FP32 instructions are switched off: the compiler emits ordinary VALU FP32 ops
@@ -40,7 +40,8 @@ def build_hip(verbose: bool = False) -> str:
 
 
 def build_fortran_host(verbose: bool = False) -> str | None:
-    """amdflang host program marfa_amd/fortran/marfa_gpu (drop-in for app/main.f90)."""
+    """amdflang host programs marfa_amd/fortran/marfa_gpu (drop-in for app/main.f90) and
+    marfa_amd/fortran/marfa_par (drop-in for app/processParFile.f90)."""
     src_dir = os.path.join(HERE, "fortran")
     srcs = [os.path.join(src_dir, f) for f in ("marfa_capi.f90", "marfa_main.f90")]
     if not all(os.path.exists(s) for s in srcs):
@@ -51,13 +52,15 @@ def build_fortran_host(verbose: bool = False) -> str | None:
     out = os.path.join(src_dir, "marfa_gpu")
     mod = os.path.join(src_dir, "_mod")
     os.makedirs(mod, exist_ok=True)
-    cmd = [fc, "-O2", "-module-dir", mod, "-o", out] + srcs + [
-        "-L" + HERE, "-lmarfa", "-Wl,-rpath,$ORIGIN/.."]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("amdflang failed:\n" + r.stderr[-4000:])
+    link = ["-L" + HERE, "-lmarfa", "-Wl,-rpath,$ORIGIN/.."]
+    # marfa_gpu: drop-in for app/main.f90; marfa_par: drop-in for app/processParFile.f90
+    for exe, files in ((out, srcs), (os.path.join(src_dir, "marfa_par"), [os.path.join(src_dir, "marfa_par.f90")])):
+        cmd = [fc, "-O2", "-module-dir", mod, "-o", exe] + files + link
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stderr[-4000:])
     return out
 
 

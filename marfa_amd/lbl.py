@@ -23,7 +23,7 @@ CHI_NONE, CHI_TONKOV, CHI_POLLACK, CHI_PERRIN = 0, 1, 2, 3
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmarfa.so")
 
-ERRORS = {9: "line index outside the line list", 20: "invalid argument", 21: "inputs not set",
+ERRORS = {2: "unsupported molecule", 3: "unreadable .par field", 9: "line index outside the line list", 20: "invalid argument", 21: "inputs not set",
           22: "HIP runtime error", 23: "no HIP device", 24: "temperature outside the TIPS table"}
 
 
@@ -79,6 +79,8 @@ def library() -> C.CDLL:
     L.marfa_count_samples.argtypes = [vp, C.POINTER(_Job), i64p]
     L.marfa_grid_sums.argtypes = [vp, C.POINTER(_Job), C.c_int32, C.c_int32, dp, fp]
     L.marfa_last_timings.argtypes = [vp, fp]
+    L.marfa_species_code.argtypes = [C.c_char_p, C.c_char_p]
+    L.marfa_par_convert.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, i64p]
     _lib = L
     return L
 
@@ -89,11 +91,44 @@ EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_v
            "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint", "marfa_count_samples",
            "marfa_grid_sums", "marfa_last_timings", "marfa_batch_create", "marfa_batch_destroy",
            "marfa_batch_last_error", "marfa_batch_set_tips", "marfa_batch_add_job", "marfa_batch_tune",
-           "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records")
+           "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records",
+           "marfa_species_code", "marfa_par_convert")
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- line-list ingest (host code of the library; no device needed)
+# 36-byte direct-access line record (app/processParFile.f90:176-177, read at src/LBL.f90:43)
+LINE_RECORD = np.dtype([("nu", "<f8"), ("S", "<f4"), ("gamma_air", "<f4"), ("gamma_self", "<f4"),
+                        ("elow", "<f4"), ("n_T", "<f4"), ("mol_iso", "<i4"), ("delta", "<f4")])
+
+
+def species_code(species: str) -> str:
+    """getSpeciesCode (app/processParFile.f90:189-223): 'CO2' -> '02'; raises MarfaError(2)
+    for a molecule the reference does not support."""
+    buf = C.create_string_buffer(3)
+    rc = library().marfa_species_code(species.encode(), buf)
+    if rc:
+        raise MarfaError(rc, f"unsupported molecule: {species.strip()}")
+    return buf.value.decode()
+
+
+def par_to_db(species: str, par_path: str, db_path: str) -> int:
+    """Convert a HITRAN .par file into the 36-byte line records (the reference's program
+    app/processParFile.f90); returns the number of records written.  Errors as the
+    reference's stop codes: 2 unsupported molecule, 3 unreadable field."""
+    n = C.c_int64(0)
+    rc = library().marfa_par_convert(species.encode(), os.fsencode(par_path), os.fsencode(db_path), C.byref(n))
+    if rc:
+        raise MarfaError(rc, f"{par_path} -> {db_path}")
+    return int(n.value)
+
+
+def read_db(db_path: str) -> np.ndarray:
+    """The 36-byte line records of a database file as a structured array (LINE_RECORD)."""
+    return np.fromfile(db_path, dtype=LINE_RECORD)
 
 
 @dataclass

@@ -12,6 +12,8 @@
 #   oracle/_ref/marfa_ref   the reference's own program (app/main.f90)
 #   oracle/_ref/ref_probe   oracle/ref_probe.f90 over the reference modules
 #   oracle/_ref/ref_driver  oracle/ref_driver.f90 over the reference modules
+#   oracle/_ref/processParFile  the reference's .par preprocessing program
+#                               (app/processParFile.f90)
 set -euo pipefail
 REF=${MARFA_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)
@@ -37,4 +39,5 @@ $FC $FFLAGS -c "$HERE/ref_probe.f90" -o ref_probe.o -module-dir "$OUT" -I "$OUT"
 $FC $FFLAGS -o ref_probe $OBJS ref_probe.o
 $FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o -module-dir "$OUT" -I "$OUT"
 $FC $FFLAGS -o ref_driver $OBJS ref_driver.o
-echo "build_ref.sh: built $OUT/marfa_ref $OUT/ref_probe $OUT/ref_driver"
+$FC $FFLAGS -o processParFile "$REF/app/processParFile.f90" -module-dir "$OUT"
+echo "build_ref.sh: built $OUT/marfa_ref $OUT/ref_probe $OUT/ref_driver $OUT/processParFile"
