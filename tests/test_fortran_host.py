@@ -136,6 +136,12 @@ def test_host_matches_c_abi_and_oracle(tmp_path, target, chi, prec):
     finally:
         ctx.close()
     assert np.array_equal(host, ref.reshape(host.shape))
+    # the Python writer (marfa_amd/pttable.py) produces the same files from the same records
+    from marfa_amd import pttable
+    base = tmp_path / "output/ptTables"
+    sub = base / (base / "latest_run.txt").read_text().strip()
+    for p in pttable.write_pt_tables(tmp_path / "py", ref.reshape(host.shape), recnums):
+        assert p.read_bytes() == (sub / p.name).read_bytes(), p.name
 
     oj = po.OracleJob(job, tips)
     o, _ = oj.pt_tables(po.F64)
