@@ -1,17 +1,17 @@
 // lbl_fine.h -- grids 1..9 and the fine points of one 1 cm-1 tile, by gather, then the
 // tile's cascade.  Included by marfa_lbl.hip inside its kernel namespace.
 //
-// Every grid node of the tile is owned by exactly one thread (grids 5..9, fine points) or
-// one wavefront (grids 1..4, whose nodes see hundreds of lines); the owner sums the
-// lines that deposit into it in the order of their shifted centres, so no sum depends on
-// scheduling and no atomics are used.  The lines depositing into a node are found in
-// O(1): a per-level index of the sorted shifted centres in 1/512 cm-1 buckets gives the
-// candidate range, and the reference's walk is inverted in closed form for the exact
-// test.  Grids 5..9 and the fine points read their centre lines from an LDS copy of the
-// tile's reach (u and sorted index, with a local bucket table); grids 1..4, whose reach
-// spans several tiles, and the wing lines read the level's sorted arrays directly.  In the
-// FP32 form the next line's shape parameters are loaded while the current line is
-// evaluated.
+// Every sum of the tile has one owner, which adds its lines in the order of their shifted
+// centres, so no sum depends on scheduling and no atomics are used.  Grids 1..9 are summed
+// by cell pairs: the lines of one walk group deposit into one pair of cells and nowhere
+// else on that grid (pair_walk), so a pair's six sums are owned by a set of threads that
+// split its lines and meet in a fixed butterfly.  Fine points are owned by threads, four
+// per grid-9 cell, which share one candidate range.  Candidate ranges come from a
+// per-level index of the sorted shifted centres in 1/512 cm-1 buckets; grids 4..9 and the
+// fine points read their centre lines from an LDS copy of the tile's reach (u and sorted
+// index, with a local bucket table), grids 1..3 (whose groups reach up to 2 cm-1 beyond
+// the tile) and the wing lines read the level's sorted line records.  The next line's
+// record is loaded while the current line is evaluated.
 //
 // For a line at u = nu* - a (src/LineGridCalc.f90:191) the left walk of centerLBL
 // (:201-331) covers, on grid k, the cells (2 e_{k-1}, e_k] with e_k = max(0, floor(u 2^k) - 1);
@@ -133,11 +133,11 @@ struct GlobalSrc {
 
 // Centre lines (class 2) of one tile's fine reach, staged in LDS in sorted order: u = nu* - org
 // and the sorted index, plus a local bucket table s_b[b] = first staged line with
-// u >= ustart + b/512.  Grids 5..9 and the fine points of a tile [t, t+1) cm-1 only see
-// centre lines with u in [t - 4 H_5, t + 1 + 4 H_5) (left walk of grid k: u in
-// [x + H_k, x + 4 H_k); right walk mirrored; fine core within 2 H_9 of the point), so
-// the staged reach [t - STAGE_REACH, t + 1 + STAGE_REACH) holds every line they need.
-constexpr double STAGE_REACH = 0.1328125;  // 4 H_5 + 4/512, exact in binary
+// u >= ustart + b/512.  The cell pairs of grid k >= 4 in a tile [t, t+1) cm-1 take centre
+// lines with u in (t - H_{k-1}, t + 1 + H_{k-1}) (H_3 = 0.125) and the fine points lines
+// within 2 H_9 + H of the point, so the staged reach [t - STAGE_REACH, t + 1 + STAGE_REACH)
+// holds every line they need.
+constexpr double STAGE_REACH = 0.1328125;  // H_3 + 4/512, exact in binary
 constexpr int STAGE_NB = 648;              // (1 + 2 STAGE_REACH) * 512 buckets
 constexpr int STAGE_CAP = 1024;            // default staged lines; denser tiles read global memory
 
@@ -501,7 +501,7 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t) {
     }
 }
 
-// One (record, tile) work item w by node-owner gathers: grid 1..9 sums into g, cascade,
+// One (record, tile) work item w by owner gathers: grid 1..9 sums into g, cascade,
 // fine points in registers, record write.  LDS: g[NACC], staging arrays of capacity cap.
 template <typename R>
 __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, double *s_u, int32_t *s_j, int32_t *s_b,

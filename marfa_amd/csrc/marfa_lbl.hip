@@ -6,16 +6,17 @@
 //   2. per-level counting sort of the shifted centres on 1/512 cm-1 buckets (the gather
 //      needs lines ordered by position; the bucket starts are its index).
 //   3. coarse_kernel  one workgroup per record (level, 10 cm-1 subinterval): every line of
-//      the record's window is evaluated at the 21 positions of the 1 cm-1 grid and the
-//      samples are summed per (cell, edge) in registers, then reduced across the
-//      workgroup in a fixed order.  This is the level-0 part of leftLBL / centerLBL /
+//      the record's window (in sorted order) is evaluated at the 21 positions of the
+//      1 cm-1 grid and the samples are summed per (cell, edge) in registers, then reduced
+//      across the workgroup in a fixed order.  This is the level-0 part of leftLBL / centerLBL /
 //      rightLBL (src/LineGridCalc.f90) and carries ~90% of all samples.
-//   4. fine_kernel  one workgroup per (record, 1 cm-1 tile): every node of grids 1..9 and
-//      every fine point of the tile gathers the lines that deposit into it -- the
-//      inverse of the reference's walks, in closed form -- from LDS-staged candidates
-//      sorted by position; each node is owned by one thread, so sums run in a fixed
-//      order.  The tile then runs the cascade (src/Grids.f90:80-139) in FP64 from its
-//      level-0 cell down to the 2,048 points of the record it owns.
+//   4. fine_kernel  one workgroup per (record, 1 cm-1 tile): the sums of grids 1..9 and
+//      the fine points of the tile gather the lines that deposit into them -- the inverse
+//      of the reference's walks, in closed form: on grid k the lines of one walk group
+//      deposit into one pair of cells -- from candidates sorted by position (LDS-staged
+//      near the tile); each sum has one owner and a fixed order.  The tile then runs the
+//      cascade (src/Grids.f90:80-139) in FP64 from its level-0 cell down to the 2,048
+//      points of the record it owns.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -118,54 +119,59 @@ struct SpecArgs {
     double *nus, *gD, *gL, *ST;  // [nlev][nl]
     uint64_t *keys;              // [nlev][nl] sortable bits of nus
     int32_t *idx;                // [nlev][nl]
+    int *emax;                   // [nlev] maximum exponent of S(T) (FP32 form), or null
 };
 
 __global__ __launch_bounds__(BLOCK) void spectroscopy_kernel(SpecArgs a) {
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     const int lev = blockIdx.y;
-    if (j >= a.nl) return;
-    const int64_t i = a.lmin + j;
-    const float P = a.P[lev], T = a.T[lev];
-    const double dT = T, dP = P, ps = a.pself[lev];
-    const int iso = a.molIso[i] / 100;
-    const double M = (double)bits_f(WISO_BITS[iso - 1]);
-    const double nu = a.nu[i];
-    // shifted centre: FP32 product delta*P as src/Spectroscopy.f90:48 forms it, so that
-    // every subinterval's line selection is the reference's
-    const double nus = nu + (double)(a.delta[i] * P);
-    const double gD = ((double)MARFA_DOPC_F * nus) * sqrt(dT / M);
-    const double gL = pow((double)MARFA_TREF_F / dT, (double)a.nT[i]) *
-                      ((double)a.gair[i] * (dP - ps) + (double)a.gself[i] * ps);
-    // TIPS(T) by linear interpolation, src/Spectroscopy.f90:110-114
-    const int ti = (int)((dT - 20.0) / 2.0 + 1.0);
-    const double wu = (dT - ((double)ti * 2.0 + 18.0)) / 2.0;
-    const float *row = a.tips + (iso - 1) * 501;
-    const double Q = (1.0 - wu) * (double)row[ti - 1] + wu * (double)row[ti];
-    const double c2 = (double)MARFA_C2_F, E = a.elow[i];
-    const double tf = (double)row[138] / Q;
-    const double bf = exp(-(c2 * E / dT)) / exp(-(c2 * E / (double)MARFA_TREF_F));
-    const double ef = (1.0 - exp(-((c2 * nu) / dT))) / (1.0 - exp(-((c2 * nu) / (double)MARFA_TREF_F)));
-    const double ST = (double)a.S[i] * tf * bf * ef;
-    const int64_t o = (int64_t)lev * a.nl + j;
-    a.nus[o] = nus;
-    a.gD[o] = gD;
-    a.gL[o] = gL;
-    a.ST[o] = ST;
-    a.keys[o] = key_of(nus);
-    a.idx[o] = (int32_t)i;
-}
-
-// per-level maximum exponent of S(T) (FP32 form scales samples by a power of two so the
-// strongest line sits near 1; max is order-independent, so the atomic is deterministic)
-__global__ __launch_bounds__(BLOCK) void smax_kernel(const double *ST, int64_t nl, int *emax) {
-    const int lev = blockIdx.y;
-    int e = -2000;
-    for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < nl; j += (int64_t)gridDim.x * BLOCK) {
-        double s = ST[(int64_t)lev * nl + j];
-        if (s > 0.0) e = max(e, ilogb(s));
+    int e = -2000;  // exponent of S(T) for the level maximum
+    if (j < a.nl) {
+        const int64_t i = a.lmin + j;
+        const float P = a.P[lev], T = a.T[lev];
+        const double dT = T, dP = P, ps = a.pself[lev];
+        const int iso = a.molIso[i] / 100;
+        const double M = (double)bits_f(WISO_BITS[iso - 1]);
+        const double nu = a.nu[i];
+        // shifted centre: FP32 product delta*P as src/Spectroscopy.f90:48 forms it, so that
+        // every subinterval's line selection is the reference's
+        const double nus = nu + (double)(a.delta[i] * P);
+        const double gD = ((double)MARFA_DOPC_F * nus) * sqrt(dT / M);
+        const double gL = pow((double)MARFA_TREF_F / dT, (double)a.nT[i]) *
+                          ((double)a.gair[i] * (dP - ps) + (double)a.gself[i] * ps);
+        // TIPS(T) by linear interpolation, src/Spectroscopy.f90:110-114
+        const int ti = (int)((dT - 20.0) / 2.0 + 1.0);
+        const double wu = (dT - ((double)ti * 2.0 + 18.0)) / 2.0;
+        const float *row = a.tips + (iso - 1) * 501;
+        const double Q = (1.0 - wu) * (double)row[ti - 1] + wu * (double)row[ti];
+        const double c2 = (double)MARFA_C2_F, E = a.elow[i];
+        const double tf = (double)row[138] / Q;
+        const double bf = exp(-(c2 * E / dT)) / exp(-(c2 * E / (double)MARFA_TREF_F));
+        const double ef = (1.0 - exp(-((c2 * nu) / dT))) / (1.0 - exp(-((c2 * nu) / (double)MARFA_TREF_F)));
+        const double ST = (double)a.S[i] * tf * bf * ef;
+        const int64_t o = (int64_t)lev * a.nl + j;
+        a.nus[o] = nus;
+        a.gD[o] = gD;
+        a.gL[o] = gL;
+        a.ST[o] = ST;
+        a.keys[o] = key_of(nus);
+        a.idx[o] = (int32_t)i;
+        if (ST > 0.0) e = ilogb(ST);
     }
-    for (int off = 32; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(emax + lev, e);
+    if (a.emax) {
+        // per-level maximum exponent of S(T) for the FP32 form's sample scale: block maximum,
+        // then an atomic only when it exceeds the value already there (the maximum only
+        // grows, so a stale read merely costs an atomic; max is order-independent, so the
+        // result is deterministic)
+        __shared__ int wmax[BLOCK / 64];
+        for (int off = 32; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = e;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < BLOCK / 64; w++) e = max(e, wmax[w]);
+            if (e > *(volatile const int *)(a.emax + lev)) atomicMax(a.emax + lev, e);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ 3. coarse (level 0)
@@ -691,14 +697,10 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         sa.ST = (double *)c->s->params[3].p;
         sa.keys = (uint64_t *)c->s->keys[0].p;
         sa.idx = (int32_t *)c->s->idx[0].p;
+        sa.emax = job.precision == MARFA_FP32K ? demax + l0 : nullptr;
         dim3 gs((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
         hipLaunchKernelGGL(spectroscopy_kernel, gs, dim3(BLOCK), 0, st, sa);
         HIPCHK(hipGetLastError());
-        if (job.precision == MARFA_FP32K) {
-            dim3 gm((unsigned)std::min<int64_t>((nl + BLOCK - 1) / BLOCK, 1024), (unsigned)nb);
-            hipLaunchKernelGGL(smax_kernel, gm, dim3(BLOCK), 0, st, (const double *)sa.ST, nl, demax + l0);
-            HIPCHK(hipGetLastError());
-        }
         hipLaunchKernelGGL(scale_kernel, dim3((nb + 63) / 64), dim3(64), 0, st, (const int *)demax + l0, nb, dsc + l0);
         HIPCHK(hipGetLastError());
         if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
