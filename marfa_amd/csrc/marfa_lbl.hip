@@ -244,11 +244,10 @@ __device__ inline void chi_law(int b, R &coef, R &rate) {
 // precision R from d >= 1 (relative error ~1 ulp).  When chi keeps one band over those
 // nodes (Tonkov / Pollack), chi(|X| +- 1/2) = chi(|X|) exp(-+rate/2): chi is evaluated at
 // the first node and carried to the others by that FP64 ratio, one exp per line instead of
-// one per sample.  Every other line takes the masked per-node path with X = u - q/2.
-// Both forms run at 2 waves per SIMD: the 31 FP64 node sums and the three unrolled node
-// loops do not fit the register budget of 3 without spilling the sums.
+// one per sample.  The lines closer than 1 cm-1 take the masked per-node path with
+// X = u - q/2.  The FP32 form runs at 3 waves per SIMD, the FP64 form at 2.
 template <typename R, int CHI>
-__global__ __launch_bounds__(BLOCK, 2) void coarse_kernel(KArgs a) {
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(KArgs a) {
     constexpr bool kBands = CHI == 1 || CHI == 2;
     __shared__ double rho_tab[8];  // exp(-rate_b * step), step = +1/2 (even entries), -1/2
     if (kBands && threadIdx.x < 8) {
@@ -279,43 +278,45 @@ __global__ __launch_bounds__(BLOCK, 2) void coarse_kernel(KArgs a) {
         J1 = b1 > b0 ? bst[b1] : J0;
     }
 
-    Acc aR, aP[10], aM[10], aL[10];
-    aR.zero();
-#pragma unroll
-    for (int c = 0; c < 10; c++) { aP[c].zero(); aM[c].zero(); aL[c].zero(); }
-    // node q: position q/2 of the subinterval; even q is L of cell q/2 and P of cell q/2+1
-    auto add_all = [&](int q, double val) {
-        if (q == 0) aR.add(val);
-        if (q & 1) {
-            aM[q >> 1].add(val);
-        } else {
-            if (q <= 18) aP[q >> 1].add(val);
-            if (q >= 2) aL[(q >> 1) - 1].add(val);
-        }
-    };
+    // The lines at least 1 cm-1 outside the subinterval (every sorted line before Ja has
+    // u < -1, every line from Jb on u > 11: bucket bounds with one bucket of margin) deposit
+    // into every node with every role, so they are summed per position q first (21 sums);
+    // the lines in between -- centre lines and wings closer than 1 cm-1 -- then add with
+    // their roles into the 31 node sums.  The two phases keep 21 and 31 sums live, not both.
+    int Ja, Jb;
+    {
+        const double xa = (org - 1.0 - a.bbase) * BUCKETS_PER_CM, xb = (org + 11.0 - a.bbase) * BUCKETS_PER_CM;
+        const int ba = (int)fmin(fmax(floor(xa) - 1.0, 0.0), (double)a.NB);
+        const int bb = (int)fmin(fmax(floor(xb) + 2.0, 0.0), (double)a.NB);
+        Ja = min(max(bst[ba], J0), J1);
+        Jb = min(max(bst[bb], Ja), J1);
+    }
+    const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
 
-    for (int j = J0 + threadIdx.x; j < J1; j += BLOCK) {
+    Acc pos[21];
+#pragma unroll
+    for (int q = 0; q <= 20; q++) pos[q].zero();
+    const int nleft = Ja - J0, nfar = nleft + (J1 - Jb);
+    for (int v = threadIdx.x; v < nfar; v += BLOCK) {
+        const int j = v < nleft ? J0 + v : Jb + (v - nleft);
         const LineShape<R> p = shp[j];
         if (p.idx < lo || p.idx >= hi) continue;  // the record's window (unshifted centres)
-        const double nus = p.nus;
-        const double u = nus - org;
-        uint32_t mP, mM, mL;
-        if (!level0_masks(nus, org, cut, mP, mM, mL)) continue;
-        // a line at least 1 cm-1 outside the subinterval deposits into every level-0 node
-        // (full masks); when even its nearest node, |X| >= dmin, is on the Lorentz branch
-        // (|X| xs >= margin as shape_eval tests it; rounding is monotone, so every node is),
-        // the nodes are summed without masks or per-sample branch tests
-        const double dmin = -u >= 1.0 ? -u : u - 10.0;
-        const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
-        if (dmin >= 1.0 && (ctl.shape_id == 1 || (R)dmin * p.xs >= margin)) {
-            const bool lw = -u >= 1.0;
+        const double u = p.nus - org;
+        const bool lw = p.nus < org;
+        // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
+        const double dmin = lw ? -u : u - 10.0;
+        if (lw ? dmin > cut : dmin >= cut) continue;
+        // every node, every role: when even the nearest node, |X| >= dmin, is on the Lorentz
+        // branch (|X| xs >= margin as shape_eval tests it; rounding is monotone, so every
+        // node is), the nodes are summed without per-sample branch tests
+        if (ctl.shape_id == 1 || (R)dmin * p.xs >= margin) {
             const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
             const R step = lw ? R(0.5) : R(-0.5);
             if (nochi) {
 #pragma unroll
                 for (int q = 0; q <= 20; q++) {
                     const R ax = q == 0 ? base : fma(step, (R)q, base);
-                    add_all(q, (double)(p.lorc * recip<R>(ax * ax + p.gl2)));
+                    pos[q].add((double)(p.lorc * recip<R>(ax * ax + p.gl2)));
                 }
                 continue;
             }
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(BLOCK, 2) void coarse_kernel(KArgs a) {
                     for (int q = 0; q <= 20; q++) {
                         const R ax = q == 0 ? base : fma(step, (R)q, base);
                         if (q > 0) chi *= rho;
-                        add_all(q, (double)(p.lorc * recip<R>(ax * ax + p.gl2)) * chi);
+                        pos[q].add((double)(p.lorc * recip<R>(ax * ax + p.gl2)) * chi);
                     }
                     continue;
                 }
@@ -338,10 +339,30 @@ __global__ __launch_bounds__(BLOCK, 2) void coarse_kernel(KArgs a) {
 #pragma unroll
             for (int q = 0; q <= 20; q++) {
                 const R ax = q == 0 ? base : fma(step, (R)q, base);
-                add_all(q, (double)chilor_abs<R, CHI>(ax, p, ctl));
+                pos[q].add((double)chilor_abs<R, CHI>(ax, p, ctl));
             }
             continue;
         }
+#pragma unroll
+        for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
+    }
+
+    // node q: position q/2 of the subinterval; even q is L of cell q/2 and P of cell q/2+1
+    Acc aR, aP[10], aM[10], aL[10];
+    aR = pos[0];
+#pragma unroll
+    for (int c = 0; c < 10; c++) {
+        aP[c] = pos[2 * c];
+        aM[c] = pos[2 * c + 1];
+        aL[c] = pos[2 * c + 2];
+    }
+    for (int j = Ja + threadIdx.x; j < Jb; j += BLOCK) {
+        const LineShape<R> p = shp[j];
+        if (p.idx < lo || p.idx >= hi) continue;
+        const double nus = p.nus;
+        const double u = nus - org;
+        uint32_t mP, mM, mL;
+        if (!level0_masks(nus, org, cut, mP, mM, mL)) continue;
         const uint32_t need = mP | mM | mL | 1u;
 #pragma unroll
         for (int q = 0; q <= 20; q++) {
