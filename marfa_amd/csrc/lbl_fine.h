@@ -198,7 +198,7 @@ struct CentreLds {
 // the same pattern as a centre line's -- nodes q = 4-e from the edge, e = 0..2, plus
 // e = 3, 4 and the second cell's edge when d >= H_k (m == k) -- so they are summed in the
 // same loop.  X: left wing u - q hq, right wing (u - 10) + q' hq, as the reference forms them.
-template <typename R, bool RIGHT, typename Src>
+template <typename R, bool RIGHT, typename Src, bool WINGS_ONLY = false>
 __device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, double s[6]) {
     const double Hk = hk(k), hq = 0.5 * Hk;
     const double wlo = (double)(2 * g) * Hk, whi = (double)(2 * g + 2) * Hk;
@@ -220,6 +220,7 @@ __device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, do
             double B, C;
             bool five, plus = false;
             if (src.centre(p, u)) {
+                if (WINGS_ONLY) continue;
                 const double w = RIGHT ? 10.0 - u : u;  // distance walked from the pair's side
                 if (w < wlo || !(w < whi)) continue;
                 five = (int)floor(__builtin_ldexp(w, k)) & 1;
@@ -402,6 +403,98 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
     return staged;
 }
 
+// Grid k >= 7 of a staged tile, one line per lane.  The pairs of these grids hold a few
+// lines each (2 H_{k-1} cm-1 of lines), so a pair-per-thread split leaves most lanes idle
+// while the fullest pair's owner works.  Instead each wavefront owns a quarter of the
+// tile's pairs and walks their lines -- a contiguous range of the sorted stage per walk --
+// 64 at a time, one line per lane: each lane evaluates its line's 3 or 5 positions, and a
+// key-segmented inclusive scan across the lanes (keys = pair index, non-decreasing along
+// the sorted lines in both walks) sums every pair's values; the last lane of each segment
+// adds them to the pair's slots in LDS.  Pairs belong to one wavefront and its chunks run
+// in order, so every sum has a fixed order.  Centre lines only (wing lines are added by
+// the edge pair's walk); the wavefront's slots are zeroed first.
+template <typename R>
+__device__ inline void level_lanes(const CentreLds<R> &src, int k, int t, double *g) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double Hk = hk(k), hq = 0.5 * Hk, hk1 = 2.0 * Hk;
+    const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
+    const int pw = npair / (BLOCK / 64), p0 = wv * pw;  // this wavefront's pairs [p0, p0 + pw)
+    for (int e = lane; e < 6 * pw; e += 64) {
+        const int pi = p0 + e / 6, c = e % 6;
+        g[accb + (c % 3) * nk + 2 * pi + c / 3] = 0.0;
+    }
+    const int n = src.sb[STAGE_NB];
+    const int gl0 = t * npair + 1 + p0;  // left-walk groups [gl0, gl0 + pw)
+#pragma unroll 1
+    for (int side = 0; side < 2; side++) {
+        int j0, j1;
+        if (side == 0) {
+            j0 = src.sb[src.bk((double)gl0 * hk1)];
+            j1 = src.sb[src.bk((double)(gl0 + pw) * hk1)];
+        } else {
+            // right-walk groups (5 << k) + 1 - gl: v in [(gr_lo) H_{k-1}, (gr_hi + 1) H_{k-1})
+            const double vlo = (double)((5 << k) + 2 - gl0 - pw) * hk1, vhi = (double)((5 << k) + 2 - gl0) * hk1;
+            j0 = src.sb[src.bk(10.0 - vhi)];
+            while (j0 < n && !(10.0 - src.su[j0] < vhi)) j0++;
+            j1 = src.sb[src.bk(10.0 - vlo)];
+            while (j1 < n && !(10.0 - src.su[j1] < vlo)) j1++;
+        }
+#pragma unroll 1
+        for (int c0 = j0; c0 < j1; c0 += 64) {
+            const int j = c0 + lane;
+            int key = npair + lane;  // unique: inactive lanes form their own segments
+            R a0 = R(0), a1 = R(0), a2 = R(0), a2o = R(0), a3 = R(0), a4 = R(0);
+            if (j < j1) {
+                const LineShape<R> p = src.line(j);
+                double u;
+                src.centre(p, u);
+                const double w = side ? 10.0 - u : u;
+                const int nn = (int)floor(__builtin_ldexp(w, k));
+                const int gi = nn >> 1;
+                const int pi = side ? (5 << k) - gi - t * npair : gi - t * npair - 1;
+                if (pi >= p0 && pi < p0 + pw) {
+                    key = pi;
+                    const bool five = nn & 1;
+                    const double q0 = (double)(4 * gi - 4);
+#pragma unroll 1
+                    for (int e = 0; e < (five ? 5 : 3); e++) {
+                        const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
+                        if (e == 0) a0 = v;
+                        else if (e == 1) a1 = v;
+                        else if (e == 2) {
+                            a2 = v;
+                            if (five) a2o = v;
+                        } else if (e == 3) a3 = v;
+                        else a4 = v;
+                    }
+                }
+            }
+            // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
+            R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
+            R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int kp = __shfl_up(key, off);
+                const R u0 = __shfl_up(v0, off), u1 = __shfl_up(v1, off), u2 = __shfl_up(v2, off);
+                const R u3 = __shfl_up(v3, off), u4 = __shfl_up(v4, off), u5 = __shfl_up(v5, off);
+                if (lane >= off && kp == key) {
+                    v0 += u0; v1 += u1; v2 += u2; v3 += u3; v4 += u4; v5 += u5;
+                }
+            }
+            const int kn = __shfl_down(key, 1);
+            if (key < npair && (lane == 63 || kn != key)) {
+                const int cb = accb + 2 * key;
+                g[cb] += (double)v0;
+                g[cb + nk] += (double)v1;
+                g[cb + 2 * nk] += (double)v2;
+                g[cb + 1] += (double)v3;
+                g[cb + nk + 1] += (double)v4;
+                g[cb + 2 * nk + 1] += (double)v5;
+            }
+        }
+    }
+}
+
 // Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grid k has 2^(k-1)
 // cell pairs in the tile; the whole block works on one grid at a time, L = 256 / 2^(k-1)
 // threads per pair, so that every grid's centre lines (2 cm-1 of them: both walks) are
@@ -415,6 +508,29 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll 1
     for (int k = 1; k <= 9; k++) {
+        if (k >= 7 && staged) {
+            level_lanes<R>(lsrc, k, t, g);
+            if ((t == 0 || t == TILES - 1) && wv == (t == 0 ? 0 : BLOCK / 64 - 1)) {
+                // wing lines of the record's edge pair (not staged), by the wavefront that
+                // owns the pair, after its centre lines
+                double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                if (t == 0) pair_walk<R, true, GlobalSrc<R>, true>(gsrc, k, 5 << k, lane, 64, s);
+                else pair_walk<R, false, GlobalSrc<R>, true>(gsrc, k, 5 << k, lane, 64, s);
+                for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                    for (int c = 0; c < 6; c++) s[c] += __shfl_xor(s[c], off);
+                if (lane == 0) {
+                    const int nk = 1 << k, accb = lvl_acc_off(k), c1 = t == 0 ? 0 : nk - 2;
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        g[accb + c1 + c] += s[3 * c];
+                        g[accb + nk + c1 + c] += s[3 * c + 1];
+                        g[accb + 2 * nk + c1 + c] += s[3 * c + 2];
+                    }
+                }
+            }
+            continue;
+        }
         const int npair = 1 << (k - 1), L = BLOCK >> (k - 1);
         const int pi = tid / L, sl = tid % L;
         const int gl = t * npair + 1 + pi;         // left-walk group of the pair
