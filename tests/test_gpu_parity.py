@@ -54,13 +54,18 @@ def describe_grid_diff(g, ref, tol):
     return f"{bad.size} nodes differ; " + "; ".join(out)
 
 
-@pytest.mark.parametrize("cfg", ["plain", "wide", "chi2", "lowp"])
+@pytest.mark.parametrize("cfg", ["plain", "wide", "chi2", "lowp", "dense"])
 def test_grid_sums_fp64(ctx, tips, cfg):
+    """Every node of every grid equals the FP64 oracle's.  'dense' (1,500 lines per cm-1)
+    puts more centre lines in a tile's reach than the LDS stage holds, so its tiles take
+    the global-memory gathers on every grid."""
     from marfa_amd import lbl
     job = {"plain": small_job(),
            "wide": small_job(seed=9, n=6000, lo=3960.0, hi=4080.0, cut=25.0),
            "chi2": small_job(seed=10, n=3000, lo=3900.0, hi=4150.0, cut=125.0, chi=2),
-           "lowp": small_job(seed=11, n=2500, p0=1e-2, p1=1e-4)}[cfg]
+           "lowp": small_job(seed=11, n=2500, p0=1e-2, p1=1e-4),
+           "dense": small_job(seed=12, n=45000, lo=3995.0, hi=4025.0, a=4000.0, b=4020.0, cut=5.0,
+                              nlev=2)}[cfg]
     ctx.load_job(job, tips)
     oj = po.OracleJob(job, tips)
     spec = lbl.JobSpec.from_job(job, lbl.FP64)
