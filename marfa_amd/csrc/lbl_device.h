@@ -177,6 +177,11 @@ template <typename R> __device__ inline R recip(R x);
 template <> __device__ inline double recip<double>(double x) { return 1.0 / x; }
 template <> __device__ inline float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// a / b: the FP32 form multiplies by the hardware reciprocal (1 ulp; the FP32 form's
+// samples are held to a tolerance, not to the FP64 form's bits), every other type divides
+template <typename R> __device__ inline R fdiv(R a, R b) { return a / b; }
+template <> __device__ inline float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // Forward-mode dual number with two tangents (d/dx^2, d/dy): evaluating the Kuntz rationals
 // in it gives their exact partial derivatives for the adjoint, through the same
 // coefficient arithmetic as the value.
@@ -245,7 +250,7 @@ __device__ inline R kuntz_rational(int region, R VY, R V2) {
         num = (((E5 * V2 + D5) * V2 + C5) * V2 + B5) * V2 + A5;
         den = ((((V2 + E6) * V2 + D6) * V2 + C6) * V2 + B6) * V2 + A6;
     }
-    return num / den;
+    return fdiv<R>(num, den);
 }
 
 __constant__ static const float ASYM_U[9] = {1.f, 1.5f, 2.f, 2.5f, 3.f, 3.5f, 4.f, 4.5f, 5.f};
@@ -293,14 +298,14 @@ __device__ inline R voigt_regions(double X64, const LineShape<R> &p, const Shape
     }
     if (VX > 5.0) {
         R vx = (R)VX;
-        return chilor<R>(X, p, c) * (R(1) + R(1.5f) / (vx * vx));
+        return chilor<R>(X, p, c) * (R(1) + fdiv<R>(R(1.5f), vx * vx));
     }
     if (VX > (double)MARFA_SQRT14_F) {
         // table index decided in FP64 as the FP64 form decides it
         const int I = (int)(VX / 0.5 - (double)1.00001f);
         R vx = (R)VX;
         R F = R(2) * (R(ASYM_W[I - 1]) * (R(ASYM_U[I]) - vx) + R(ASYM_W[I]) * (vx - R(ASYM_U[I - 1])));
-        return doppler<R>(X, p) + p.lorc / (X * X) * (R(1) + F);
+        return doppler<R>(X, p) + fdiv<R>(p.lorc, X * X) * (R(1) + F);
     }
     return doppler<R>(X, p);
 }
