@@ -403,93 +403,108 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
     return staged;
 }
 
-// Grid k >= 7 of a staged tile, one line per lane.  The pairs of these grids hold a few
-// lines each (2 H_{k-1} cm-1 of lines), so a pair-per-thread split leaves most lanes idle
-// while the fullest pair's owner works.  Instead each wavefront owns a quarter of the
-// tile's pairs and walks their lines -- a contiguous range of the sorted stage per walk --
-// 64 at a time, one line per lane: each lane evaluates its line's 3 or 5 positions, and a
-// key-segmented inclusive scan across the lanes (keys = pair index, non-decreasing along
-// the sorted lines in both walks) sums every pair's values; the last lane of each segment
-// adds them to the pair's slots in LDS.  Pairs belong to one wavefront and its chunks run
-// in order, so every sum has a fixed order.  Centre lines only (wing lines are added by
-// the edge pair's walk); the wavefront's slots are zeroed first.
+// Grids LANES_K0..9 of a staged tile, one line per lane.  The pairs of these grids hold a
+// few lines each (2 H_{k-1} cm-1 of lines), so a pair-per-thread split leaves most lanes
+// idle while the fullest pair's owner works.  Instead each wavefront owns a quarter of every
+// grid's pairs and walks their lines -- per walk a contiguous range of the sorted stage --
+// 64 at a time, one line per lane.  A wavefront's pairs of grid k take, on the left walk,
+// u in [t + wv/4, t + (wv+1)/4) + H_{k-1} (on the right walk, the mirror image), so one
+// pass over the union of these ranges serves all three grids: each lane loads its line
+// once and, grid after grid, evaluates its 3 or 5 positions when the line's pair (the same
+// floor as the deposit geometry) is one of the wavefront's; a key-segmented inclusive scan
+// across the lanes (keys = pair index, non-decreasing along the sorted lines in both walks;
+// lanes with no pair on that grid have unique keys) sums every pair's values, and the last
+// lane of each segment adds them to the pair's slots in LDS.  Pairs belong to one
+// wavefront and its chunks run in order, so every sum has a fixed order.  Centre lines
+// only (wing lines are added by the edge pair's walk); the wavefront's slots are zeroed
+// first.
+constexpr int LANES_K0 = 7;
+
 template <typename R>
-__device__ inline void level_lanes(const CentreLds<R> &src, int k, int t, double *g) {
+__device__ inline void level_lanes(const CentreLds<R> &src, int t, double *g) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const double Hk = hk(k), hq = 0.5 * Hk, hk1 = 2.0 * Hk;
-    const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
-    const int pw = npair / (BLOCK / 64), p0 = wv * pw;  // this wavefront's pairs [p0, p0 + pw)
-    for (int e = lane; e < 6 * pw; e += 64) {
-        const int pi = p0 + e / 6, c = e % 6;
-        g[accb + (c % 3) * nk + 2 * pi + c / 3] = 0.0;
+#pragma unroll 1
+    for (int k = LANES_K0; k <= 9; k++) {
+        const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
+        const int pw = npair / (BLOCK / 64), p0 = wv * pw;  // this wavefront's pairs [p0, p0 + pw)
+        for (int e = lane; e < 6 * pw; e += 64) {
+            const int pi = p0 + e / 6, c = e % 6;
+            g[accb + (c % 3) * nk + 2 * pi + c / 3] = 0.0;
+        }
     }
-    const int n = src.sb[STAGE_NB];
-    const int gl0 = t * npair + 1 + p0;  // left-walk groups [gl0, gl0 + pw)
+    const double lo = (double)t + 0.25 * wv, hi = lo + 0.25;
 #pragma unroll 1
     for (int side = 0; side < 2; side++) {
         int j0, j1;
-        if (side == 0) {
-            j0 = src.sb[src.bk((double)gl0 * hk1)];
-            j1 = src.sb[src.bk((double)(gl0 + pw) * hk1)];
-        } else {
-            // right-walk groups (5 << k) + 1 - gl: v in [(gr_lo) H_{k-1}, (gr_hi + 1) H_{k-1})
-            const double vlo = (double)((5 << k) + 2 - gl0 - pw) * hk1, vhi = (double)((5 << k) + 2 - gl0) * hk1;
-            j0 = src.sb[src.bk(10.0 - vhi)];
-            while (j0 < n && !(10.0 - src.su[j0] < vhi)) j0++;
-            j1 = src.sb[src.bk(10.0 - vlo)];
-            while (j1 < n && !(10.0 - src.su[j1] < vlo)) j1++;
+        if (side == 0) {  // exactly u in [lo + H_8, hi + H_{K0-1}): bounds on the bucket grid
+            j0 = src.sb[src.bk(lo + hk(8))];
+            j1 = src.sb[src.bk(hi + hk(LANES_K0 - 1))];
+        } else {          // u in (lo - H_{K0-1}, hi - H_8], one bucket wider (w = fl(10 - u))
+            j0 = src.sb[src.bk(lo - hk(LANES_K0 - 1)) - 1];
+            j1 = src.sb[src.bk(hi - hk(8)) + 1];
         }
 #pragma unroll 1
         for (int c0 = j0; c0 < j1; c0 += 64) {
             const int j = c0 + lane;
-            int key = npair + lane;  // unique: inactive lanes form their own segments
-            R a0 = R(0), a1 = R(0), a2 = R(0), a2o = R(0), a3 = R(0), a4 = R(0);
-            if (j < j1) {
-                const LineShape<R> p = src.line(j);
+            const bool act = j < j1;
+            LineShape<R> p;
+            double w = 0.0;
+            if (act) {
+                p = src.line(j);
                 double u;
                 src.centre(p, u);
-                const double w = side ? 10.0 - u : u;
-                const int nn = (int)floor(__builtin_ldexp(w, k));
-                const int gi = nn >> 1;
-                const int pi = side ? (5 << k) - gi - t * npair : gi - t * npair - 1;
-                if (pi >= p0 && pi < p0 + pw) {
-                    key = pi;
-                    const bool five = nn & 1;
-                    const double q0 = (double)(4 * gi - 4);
+                w = side ? 10.0 - u : u;
+            }
 #pragma unroll 1
-                    for (int e = 0; e < (five ? 5 : 3); e++) {
-                        const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
-                        if (e == 0) a0 = v;
-                        else if (e == 1) a1 = v;
-                        else if (e == 2) {
-                            a2 = v;
-                            if (five) a2o = v;
-                        } else if (e == 3) a3 = v;
-                        else a4 = v;
+            for (int k = LANES_K0; k <= 9; k++) {
+                const double hq = 0.5 * hk(k);
+                const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
+                const int pw = npair / (BLOCK / 64), p0 = wv * pw;
+                int key = npair + lane;  // unique: lanes without a pair here form their own segments
+                R a0 = R(0), a1 = R(0), a2 = R(0), a2o = R(0), a3 = R(0), a4 = R(0);
+                if (act) {
+                    const int nn = (int)floor(__builtin_ldexp(w, k));
+                    const int gi = nn >> 1;
+                    const int pi = side ? (5 << k) - gi - t * npair : gi - t * npair - 1;
+                    if (pi >= p0 && pi < p0 + pw) {
+                        key = pi;
+                        const bool five = nn & 1;
+                        const double q0 = (double)(4 * gi - 4);
+#pragma unroll 1
+                        for (int e = 0; e < (five ? 5 : 3); e++) {
+                            const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
+                            if (e == 0) a0 = v;
+                            else if (e == 1) a1 = v;
+                            else if (e == 2) {
+                                a2 = v;
+                                if (five) a2o = v;
+                            } else if (e == 3) a3 = v;
+                            else a4 = v;
+                        }
                     }
                 }
-            }
-            // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
-            R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
-            R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
+                // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
+                R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
+                R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int kp = __shfl_up(key, off);
-                const R u0 = __shfl_up(v0, off), u1 = __shfl_up(v1, off), u2 = __shfl_up(v2, off);
-                const R u3 = __shfl_up(v3, off), u4 = __shfl_up(v4, off), u5 = __shfl_up(v5, off);
-                if (lane >= off && kp == key) {
-                    v0 += u0; v1 += u1; v2 += u2; v3 += u3; v4 += u4; v5 += u5;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int kp = __shfl_up(key, off);
+                    const R u0 = __shfl_up(v0, off), u1 = __shfl_up(v1, off), u2 = __shfl_up(v2, off);
+                    const R u3 = __shfl_up(v3, off), u4 = __shfl_up(v4, off), u5 = __shfl_up(v5, off);
+                    if (lane >= off && kp == key) {
+                        v0 += u0; v1 += u1; v2 += u2; v3 += u3; v4 += u4; v5 += u5;
+                    }
                 }
-            }
-            const int kn = __shfl_down(key, 1);
-            if (key < npair && (lane == 63 || kn != key)) {
-                const int cb = accb + 2 * key;
-                g[cb] += (double)v0;
-                g[cb + nk] += (double)v1;
-                g[cb + 2 * nk] += (double)v2;
-                g[cb + 1] += (double)v3;
-                g[cb + nk + 1] += (double)v4;
-                g[cb + 2 * nk + 1] += (double)v5;
+                const int kn = __shfl_down(key, 1);
+                if (key < npair && (lane == 63 || kn != key)) {
+                    const int cb = accb + 2 * key;
+                    g[cb] += (double)v0;
+                    g[cb + nk] += (double)v1;
+                    g[cb + 2 * nk] += (double)v2;
+                    g[cb + 1] += (double)v3;
+                    g[cb + nk + 1] += (double)v4;
+                    g[cb + 2 * nk + 1] += (double)v5;
+                }
             }
         }
     }
@@ -508,8 +523,8 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll 1
     for (int k = 1; k <= 9; k++) {
-        if (k >= 7 && staged) {
-            level_lanes<R>(lsrc, k, t, g);
+        if (k >= LANES_K0 && staged) {
+            if (k == LANES_K0) level_lanes<R>(lsrc, t, g);
             if ((t == 0 || t == TILES - 1) && wv == (t == 0 ? 0 : BLOCK / 64 - 1)) {
                 // wing lines of the record's edge pair (not staged), by the wavefront that
                 // owns the pair, after its centre lines
