@@ -5,7 +5,9 @@
 // centres, so no sum depends on scheduling and no atomics are used.  Grids 1..9 are summed
 // by cell pairs: the lines of one walk group deposit into one pair of cells and nowhere
 // else on that grid (pair_walk), so a pair's six sums are owned by a set of threads that
-// split its lines and meet in a fixed butterfly.  Fine points are owned by threads, four
+// split its lines and meet in a fixed butterfly; on grids 6..9 of a staged tile a wavefront
+// owns a quarter of the pairs and takes one line per lane for all four grids at once
+// (level_lanes), summing each pair's lines by a lane scan.  Fine points are owned by threads, four
 // per grid-9 cell, which share one candidate range.  Candidate ranges come from a
 // per-level index of the sorted shifted centres in 1/512 cm-1 buckets; grids 4..9 and the
 // fine points read their centre lines from an LDS copy of the tile's reach (u and sorted
@@ -418,7 +420,7 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
 // wavefront and its chunks run in order, so every sum has a fixed order.  Centre lines
 // only (wing lines are added by the edge pair's walk); the wavefront's slots are zeroed
 // first.
-constexpr int LANES_K0 = 7;
+constexpr int LANES_K0 = 6;
 
 template <typename R>
 __device__ inline void level_lanes(const CentreLds<R> &src, int t, double *g) {
