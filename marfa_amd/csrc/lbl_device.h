@@ -104,7 +104,7 @@ struct LineShape {
     R normc;  // S / sqrt(pi) / (gD/sqrt(ln2)) : K(x,y) -> profile
     R dopa;   // sqrt(ln2)/(sqrt(pi) gD) S    : Doppler amplitude
     R igd;    // 1/gD
-    R gL, S;
+    R S;      // line intensity (the adjoint's dF/dgL)
     // region decisions in FP64: VX = |X| xs64 (within 2 ulp of the FP64 form's
     // |sqrt(ln2) X / gD|), VY = vy64 formed as the FP64 form forms it
     double xs64, vy64;
@@ -113,6 +113,7 @@ struct LineShape {
     double nus;
     int32_t idx;
 };
+static_assert(sizeof(LineShape<float>) == 64 && sizeof(LineShape<double>) == 96, "one line record per load");
 
 template <typename R>
 __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
@@ -126,7 +127,6 @@ __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
         p.normc = S / (double)MARFA_SQRTPI_F / (gD / sq);
         p.dopa = sq / ((double)MARFA_SQRTPI_F * gD) * S;
         p.igd = 1.0 / gD;
-        p.gL = gL;
         p.S = S;
     } else {
         const float g = (float)gD, l = (float)gL, s = (float)S;
@@ -137,7 +137,6 @@ __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
         p.normc = s / MARFA_SQRTPI_F / (g / MARFA_SQLN2_F);
         p.dopa = MARFA_SQLN2_F / (MARFA_SQRTPI_F * g) * s;
         p.igd = 1.0f / g;
-        p.gL = l;
         p.S = s;
     }
     p.xs64 = sq / gD;

@@ -14,14 +14,61 @@ __global__ __launch_bounds__(BLOCK) void fill_i32_kernel(int32_t *p, int64_t n, 
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) p[i] = v;
 }
 
+// Consecutive lines of a level are nearly in order of shifted centre (the line list is
+// sorted by centre; the pressure shift moves each by a fraction of a cm-1), so the
+// BLOCK lines of one workgroup usually fall into a short run of buckets.  When that run is
+// shorter than HIST_SPAN the workgroup counts in LDS and updates each non-empty bucket
+// with one global atomic; otherwise every line updates its bucket (the integer counts are
+// the same either way).
+constexpr int HIST_SPAN = 2048;
+
+// bucket of line j of level lev (act: j < nl) and the workgroup's [bmin, bmax] over its
+// active lines (bmin > bmax when none is active)
+__device__ inline int block_buckets(const uint64_t *keys, int64_t nl, double base, int NB, int lev, int64_t j,
+                                    bool act, int *red, int &bmin, int &bmax) {
+    const int b = act ? bucket_of(double_of(keys[(int64_t)lev * nl + j]), base, NB) : 0;
+    int lo = act ? b : INT32_MAX, hi = act ? b : -1;
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wv] = lo;
+        red[BLOCK / 64 + wv] = hi;
+    }
+    __syncthreads();
+    bmin = red[0];
+    bmax = red[BLOCK / 64];
+    for (int w = 1; w < BLOCK / 64; w++) {
+        bmin = min(bmin, red[w]);
+        bmax = max(bmax, red[BLOCK / 64 + w]);
+    }
+    return b;
+}
+
 // cnt[lev][b] += 1 for every line of the level (cnt zero on entry)
 __global__ __launch_bounds__(BLOCK) void bucket_hist_kernel(const uint64_t *keys, int64_t nl, double base, int NB,
                                                             int32_t *cnt) {
+    __shared__ int h[HIST_SPAN];
+    __shared__ int red[2 * (BLOCK / 64)];
     const int lev = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= nl) return;
-    const int b = bucket_of(double_of(keys[(int64_t)lev * nl + j]), base, NB);
-    atomicAdd(cnt + (int64_t)lev * NB + b, 1);
+    const bool act = j < nl;
+    int bmin, bmax;
+    const int b = block_buckets(keys, nl, base, NB, lev, j, act, red, bmin, bmax);
+    int32_t *c = cnt + (int64_t)lev * NB;
+    if (bmax - bmin >= HIST_SPAN) {
+        if (act) atomicAdd(c + b, 1);
+        return;
+    }
+    const int span = bmax - bmin + 1;
+    for (int i = threadIdx.x; i < span; i += BLOCK) h[i] = 0;
+    __syncthreads();
+    if (act) atomicAdd(h + (b - bmin), 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < span; i += BLOCK)
+        if (h[i]) atomicAdd(c + bmin + i, h[i]);
 }
 
 __device__ inline int block_exclusive_scan(int x, int *tmp, int &total) {
@@ -97,19 +144,43 @@ __global__ __launch_bounds__(BLOCK) void scan_final_kernel(const int32_t *cnt, i
 }
 
 // place every line in its bucket's range (order inside the bucket set by the atomics);
-// cnt is consumed (back to zero)
+// cnt is consumed (back to zero).  A workgroup whose lines span fewer than HIST_SPAN
+// buckets ranks its lines per bucket in LDS and takes each bucket's slots with one global
+// atomic: its lines of bucket b get positions bst[b] + [old - n, old), old = the count
+// before the atomic, n = its lines in b.
 __global__ __launch_bounds__(BLOCK) void bucket_scatter_kernel(const uint64_t *keys, const int32_t *idx, int64_t nl,
                                                                double base, int NB, const int32_t *bst, int32_t *cnt,
                                                                uint64_t *skey, int32_t *sidx) {
+    __shared__ int h[HIST_SPAN];
+    __shared__ int red[2 * (BLOCK / 64)];
     const int lev = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= nl) return;
+    const bool act = j < nl;
+    int bmin, bmax;
+    const int b = block_buckets(keys, nl, base, NB, lev, j, act, red, bmin, bmax);
     const int64_t o = (int64_t)lev * nl;
-    const uint64_t k = keys[o + j];
-    const int b = bucket_of(double_of(k), base, NB);
-    const int pos = bst[(int64_t)lev * (NB + 1) + b] + atomicSub(cnt + (int64_t)lev * NB + b, 1) - 1;
-    skey[o + pos] = k;
-    sidx[o + pos] = idx[o + j];
+    int32_t *c = cnt + (int64_t)lev * NB;
+    const int32_t *s0 = bst + (int64_t)lev * (NB + 1);
+    int pos = 0;
+    if (bmax - bmin >= HIST_SPAN) {
+        if (act) pos = s0[b] + atomicSub(c + b, 1) - 1;
+    } else {
+        const int span = bmax - bmin + 1;
+        for (int i = threadIdx.x; i < span; i += BLOCK) h[i] = 0;
+        __syncthreads();
+        const int r = act ? atomicAdd(h + (b - bmin), 1) : 0;  // rank among the workgroup's lines of b
+        __syncthreads();
+        for (int i = threadIdx.x; i < span; i += BLOCK) {
+            const int n = h[i];
+            if (n) h[i] = atomicSub(c + bmin + i, n) - n;  // first slot of the workgroup's lines of b
+        }
+        __syncthreads();
+        if (act) pos = s0[b] + h[b - bmin] + r;
+    }
+    if (act) {
+        skey[o + pos] = keys[o + j];
+        sidx[o + pos] = idx[o + j];
+    }
 }
 
 // insertion sort of each bucket by (key, line index): the stable order by shifted centre
