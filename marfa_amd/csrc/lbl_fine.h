@@ -31,24 +31,6 @@ __device__ inline int bucket_of(double nus, double base, int NB) {
     return (int)x;
 }
 
-// profile parameters of every line, in sorted order, for the fine gathers
-template <typename R>
-__global__ __launch_bounds__(BLOCK) void shape_kernel(const int32_t *sidx, int64_t nl, int64_t lmin,
-                                                      const double *nus, const double *gD, const double *gL,
-                                                      const double *S, const double *sscale, int lev0,
-                                                      LineShape<R> *shp) {
-    const int lev = blockIdx.y;
-    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= nl) return;
-    const int64_t o = (int64_t)lev * nl;
-    const int32_t li = sidx[o + j];
-    const int64_t i = o + (li - lmin);
-    LineShape<R> p = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
-    p.nus = nus[i];
-    p.idx = li;
-    shp[o + j] = p;
-}
-
 template <typename R>
 struct FineCtx {
     const uint64_t *skey;

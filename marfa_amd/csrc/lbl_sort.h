@@ -2,10 +2,11 @@
 // Included by marfa_lbl.hip inside its kernel namespace.
 //
 // A counting sort on the 1/512 cm-1 buckets the fine gathers use: histogram (integer
-// atomics), exclusive scan (the bucket index bst itself), scatter, then an insertion sort
-// of each bucket's few lines by (shifted centre, line index).  The result is the stable
-// order by shifted centre whatever the scheduling, and every step is a plain kernel (no
-// library call, no memset), so the whole pipeline can be captured in a hipGraph.
+// atomics), exclusive scan (the bucket index bst itself), scatter, then each line's rank
+// among its bucket's few lines by (shifted centre, line index) gives its final position,
+// where its line record is written.  The result is the stable order by shifted centre
+// whatever the scheduling, and every step is a plain kernel (no library call, no memset),
+// so the whole pipeline can be captured in a hipGraph.
 #pragma once
 
 constexpr int SCAN_CHUNK = BLOCK * 16;  // buckets per scan block
@@ -183,29 +184,41 @@ __global__ __launch_bounds__(BLOCK) void bucket_scatter_kernel(const uint64_t *k
     }
 }
 
-// insertion sort of each bucket by (key, line index): the stable order by shifted centre
-__global__ __launch_bounds__(BLOCK) void bucket_order_kernel(int NB, int64_t nl, const int32_t *bst, uint64_t *skey,
-                                                             int32_t *sidx) {
+// Final order and line records: a line at position j of the scattered arrays takes
+// position bst[b] + (its rank by (key, line index) among the lines of its bucket b) --
+// the stable order by shifted centre, whatever order the atomics left inside the bucket
+// (buckets hold a few lines each).  The thread writes the sorted key and index there and
+// the line's profile parameters (LineShape: one record per sorted line, read by the
+// deposit kernels in one load).
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void order_shape_kernel(const uint64_t *keys, const int32_t *idx, int64_t nl,
+                                                            double base, int NB, const int32_t *bst, int64_t lmin,
+                                                            const double *nus, const double *gD, const double *gL,
+                                                            const double *S, const double *sscale, int lev0,
+                                                            uint64_t *skey, int32_t *sidx, LineShape<R> *shp) {
     const int lev = blockIdx.y;
-    const int b = blockIdx.x * BLOCK + threadIdx.x;
-    if (b >= NB) return;
-    const int32_t *s = bst + (int64_t)lev * (NB + 1);
-    const int lo = s[b], hi = s[b + 1];
-    if (hi - lo < 2) return;
-    uint64_t *k = skey + (int64_t)lev * nl;
-    int32_t *x = sidx + (int64_t)lev * nl;
-    for (int i = lo + 1; i < hi; i++) {
-        const uint64_t kk = k[i];
-        const int32_t xx = x[i];
-        int p = i - 1;
-        while (p >= lo && (k[p] > kk || (k[p] == kk && x[p] > xx))) {
-            k[p + 1] = k[p];
-            x[p + 1] = x[p];
-            p--;
-        }
-        k[p + 1] = kk;
-        x[p + 1] = xx;
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nl) return;
+    const int64_t o = (int64_t)lev * nl;
+    const uint64_t k = keys[o + j];
+    const int32_t x = idx[o + j];
+    const int b = bucket_of(double_of(k), base, NB);
+    const int32_t *s0 = bst + (int64_t)lev * (NB + 1);
+    const int lo = s0[b], hi = s0[b + 1];
+    int rank = 0;
+    for (int i = lo; i < hi; i++) {
+        const uint64_t kk = keys[o + i];
+        const int32_t xx = idx[o + i];
+        rank += (kk < k || (kk == k && xx < x)) ? 1 : 0;
     }
+    const int64_t pos = o + lo + rank;
+    skey[pos] = k;
+    sidx[pos] = x;
+    const int64_t i = o + (x - lmin);
+    LineShape<R> p = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
+    p.nus = nus[i];
+    p.idx = x;
+    shp[pos] = p;
 }
 
 // FP32 form: sample scale 2^-emax of every level from the per-level maximum exponent

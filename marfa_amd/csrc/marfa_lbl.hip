@@ -744,20 +744,20 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
                                nsblk, (const int32_t *)part, bst);
             hipLaunchKernelGGL(bucket_scatter_kernel, gl, dim3(BLOCK), 0, st, k0, i0, nl, bbase, NB,
                                (const int32_t *)bst, hist, (uint64_t *)c->s->keys[1].p, (int32_t *)c->s->idx[1].p);
-            hipLaunchKernelGGL(bucket_order_kernel, dim3((NB + BLOCK - 1) / BLOCK, nb), dim3(BLOCK), 0, st, NB, nl,
-                               (const int32_t *)bst, (uint64_t *)c->s->keys[1].p, (int32_t *)c->s->idx[1].p);
-            HIPCHK(hipGetLastError());
-        }
-        {
-            dim3 gsh((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
+            // final order into keys[0] / idx[0] (the unsorted copies are not read again) and
+            // the line records
             if (job.precision == MARFA_FP32K)
-                hipLaunchKernelGGL(shape_kernel<float>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->s->idx[1].p, nl,
-                                   lmin, (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
-                                   (const double *)dsc, l0, (LineShape<float> *)c->s->shapes.p);
+                hipLaunchKernelGGL(order_shape_kernel<float>, gl, dim3(BLOCK), 0, st, (const uint64_t *)c->s->keys[1].p,
+                                   (const int32_t *)c->s->idx[1].p, nl, bbase, NB, (const int32_t *)bst, lmin,
+                                   (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
+                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)c->s->keys[0].p,
+                                   (int32_t *)c->s->idx[0].p, (LineShape<float> *)c->s->shapes.p);
             else
-                hipLaunchKernelGGL(shape_kernel<double>, gsh, dim3(BLOCK), 0, st, (const int32_t *)c->s->idx[1].p, nl,
-                                   lmin, (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL, (const double *)sa.ST,
-                                   (const double *)dsc, l0, (LineShape<double> *)c->s->shapes.p);
+                hipLaunchKernelGGL(order_shape_kernel<double>, gl, dim3(BLOCK), 0, st, (const uint64_t *)c->s->keys[1].p,
+                                   (const int32_t *)c->s->idx[1].p, nl, bbase, NB, (const int32_t *)bst, lmin,
+                                   (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
+                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)c->s->keys[0].p,
+                                   (int32_t *)c->s->idx[0].p, (LineShape<double> *)c->s->shapes.p);
             HIPCHK(hipGetLastError());
         }
         if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
@@ -771,8 +771,8 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.gD = sa.gD;
         ka.gL = sa.gL;
         ka.S = sa.ST;
-        ka.skey = (const uint64_t *)c->s->keys[1].p;
-        ka.sidx = (const int32_t *)c->s->idx[1].p;
+        ka.skey = (const uint64_t *)c->s->keys[0].p;
+        ka.sidx = (const int32_t *)c->s->idx[0].p;
         ka.lo = dlo;
         ka.hi = dhi;
         ka.org = dorg;
