@@ -104,6 +104,7 @@ struct LineShape {
     R normc;  // S / sqrt(pi) / (gD/sqrt(ln2)) : K(x,y) -> profile
     R dopa;   // sqrt(ln2)/(sqrt(pi) gD) S    : Doppler amplitude
     R igd;    // 1/gD
+    R gL;     // Lorentz HWHM
     R S;      // line intensity (the adjoint's dF/dgL)
     // region decisions in FP64: VX = |X| xs64 (within 2 ulp of the FP64 form's
     // |sqrt(ln2) X / gD|), VY = vy64 formed as the FP64 form forms it
@@ -113,7 +114,10 @@ struct LineShape {
     double nus;
     int32_t idx;
 };
-static_assert(sizeof(LineShape<float>) == 64 && sizeof(LineShape<double>) == 96, "one line record per load");
+// 72 B (FP32) / 104 B (FP64).  The deposit kernels' lanes load consecutive records; a
+// 64-B record (without gL: a power-of-two stride) measured 1 ms slower in each of the two
+// deposit kernels than this layout.
+static_assert(sizeof(LineShape<float>) == 72 && sizeof(LineShape<double>) == 104, "line record layout");
 
 template <typename R>
 __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
@@ -127,6 +131,7 @@ __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
         p.normc = S / (double)MARFA_SQRTPI_F / (gD / sq);
         p.dopa = sq / ((double)MARFA_SQRTPI_F * gD) * S;
         p.igd = 1.0 / gD;
+        p.gL = gL;
         p.S = S;
     } else {
         const float g = (float)gD, l = (float)gL, s = (float)S;
@@ -137,6 +142,7 @@ __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
         p.normc = s / MARFA_SQRTPI_F / (g / MARFA_SQLN2_F);
         p.dopa = MARFA_SQLN2_F / (MARFA_SQRTPI_F * g) * s;
         p.igd = 1.0f / g;
+        p.gL = l;
         p.S = s;
     }
     p.xs64 = sq / gD;
