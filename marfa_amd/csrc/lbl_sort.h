@@ -121,25 +121,41 @@ __global__ __launch_bounds__(BLOCK) void scan_top_kernel(int32_t *part, int nblk
     }
 }
 
-// bst[lev][b] = first sorted position of bucket b (b = 0..NB; bst[NB] = nl)
+// bst[lev][b] = first sorted position of bucket b (b = 0..NB; bst[NB] = nl).  The chunk
+// passes through LDS (padded: element i at i + i/16) so that global reads and writes are
+// coalesced while each thread scans 16 consecutive buckets.
+__device__ inline int scan_pad(int i) { return i + (i >> 4); }
+
 __global__ __launch_bounds__(BLOCK) void scan_final_kernel(const int32_t *cnt, int NB, int nblk, const int32_t *part,
                                                            int32_t *bst) {
     __shared__ int tmp[BLOCK / 64];
+    __shared__ int tile[SCAN_CHUNK + SCAN_CHUNK / 16];
     const int lev = blockIdx.y, blk = blockIdx.x;
     const int32_t *c = cnt + (int64_t)lev * NB;
     int32_t *o = bst + (int64_t)lev * (NB + 1);
+    const int base = blk * SCAN_CHUNK;
+    for (int e = 0; e < 16; e++) {
+        const int i = e * BLOCK + threadIdx.x;
+        tile[scan_pad(i)] = base + i < NB ? c[base + i] : 0;
+    }
+    __syncthreads();
     // each thread owns 16 consecutive buckets of the chunk
-    const int b0 = blk * SCAN_CHUNK + threadIdx.x * 16;
+    const int i0 = threadIdx.x * 16;
     int v[16], s = 0;
     for (int e = 0; e < 16; e++) {
-        v[e] = b0 + e < NB ? c[b0 + e] : 0;
+        v[e] = tile[scan_pad(i0 + e)];
         s += v[e];
     }
     int total;
     int run = part[(int64_t)lev * nblk + blk] + block_exclusive_scan(s, tmp, total);
     for (int e = 0; e < 16; e++) {
-        if (b0 + e < NB) o[b0 + e] = run;
+        tile[scan_pad(i0 + e)] = run;
         run += v[e];
+    }
+    __syncthreads();
+    for (int e = 0; e < 16; e++) {
+        const int i = e * BLOCK + threadIdx.x;
+        if (base + i < NB) o[base + i] = tile[scan_pad(i)];
     }
     if (blk == nblk - 1 && threadIdx.x == BLOCK - 1) o[NB] = run;
 }
