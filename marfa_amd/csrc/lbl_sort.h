@@ -205,36 +205,56 @@ __global__ __launch_bounds__(BLOCK) void bucket_scatter_kernel(const uint64_t *k
 // the stable order by shifted centre, whatever order the atomics left inside the bucket
 // (buckets hold a few lines each).  The thread writes the sorted key and index there and
 // the line's profile parameters (LineShape: one record per sorted line, read by the
-// deposit kernels in one load).
+// deposit kernels in one load).  A workgroup's lines land on its own positions
+// [j0, j0 + BLOCK) except where a bucket straddles its edges, so the records go through LDS
+// and out as coalesced dwords (only the slots this workgroup filled); the few that land
+// outside are written directly.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void order_shape_kernel(const uint64_t *keys, const int32_t *idx, int64_t nl,
                                                             double base, int NB, const int32_t *bst, int64_t lmin,
                                                             const double *nus, const double *gD, const double *gL,
                                                             const double *S, const double *sscale, int lev0,
                                                             uint64_t *skey, int32_t *sidx, LineShape<R> *shp) {
+    constexpr int W = sizeof(LineShape<R>) / 4;  // dwords per record
+    __shared__ double rec[BLOCK * W / 2];
+    __shared__ int filled[BLOCK];
     const int lev = blockIdx.y;
-    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= nl) return;
+    const int64_t j0 = (int64_t)blockIdx.x * BLOCK, j = j0 + threadIdx.x;
     const int64_t o = (int64_t)lev * nl;
-    const uint64_t k = keys[o + j];
-    const int32_t x = idx[o + j];
-    const int b = bucket_of(double_of(k), base, NB);
-    const int32_t *s0 = bst + (int64_t)lev * (NB + 1);
-    const int lo = s0[b], hi = s0[b + 1];
-    int rank = 0;
-    for (int i = lo; i < hi; i++) {
-        const uint64_t kk = keys[o + i];
-        const int32_t xx = idx[o + i];
-        rank += (kk < k || (kk == k && xx < x)) ? 1 : 0;
+    filled[threadIdx.x] = 0;
+    __syncthreads();
+    if (j < nl) {
+        const uint64_t k = keys[o + j];
+        const int32_t x = idx[o + j];
+        const int b = bucket_of(double_of(k), base, NB);
+        const int32_t *s0 = bst + (int64_t)lev * (NB + 1);
+        const int lo = s0[b], hi = s0[b + 1];
+        int rank = 0;
+        for (int i = lo; i < hi; i++) {
+            const uint64_t kk = keys[o + i];
+            const int32_t xx = idx[o + i];
+            rank += (kk < k || (kk == k && xx < x)) ? 1 : 0;
+        }
+        const int64_t q = lo + rank;
+        skey[o + q] = k;
+        sidx[o + q] = x;
+        const int64_t i = o + (x - lmin);
+        LineShape<R> p = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
+        p.nus = nus[i];
+        p.idx = x;
+        if (q >= j0 && q < j0 + BLOCK) {
+            *(LineShape<R> *)(rec + (q - j0) * (W / 2)) = p;
+            filled[q - j0] = 1;
+        } else {
+            shp[o + q] = p;
+        }
     }
-    const int64_t pos = o + lo + rank;
-    skey[pos] = k;
-    sidx[pos] = x;
-    const int64_t i = o + (x - lmin);
-    LineShape<R> p = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
-    p.nus = nus[i];
-    p.idx = x;
-    shp[pos] = p;
+    __syncthreads();
+    const int n = (int)min((int64_t)BLOCK, nl - j0);
+    const uint32_t *src = (const uint32_t *)rec;
+    uint32_t *dst = (uint32_t *)(shp + o + j0);
+    for (int d = threadIdx.x; d < n * W; d += BLOCK)
+        if (filled[d / W]) dst[d] = src[d];
 }
 
 // FP32 form: sample scale 2^-emax of every level from the per-level maximum exponent
