@@ -417,71 +417,74 @@ __device__ inline void level_lanes(const CentreLds<R> &src, int t, double *g) {
         }
     }
     const double lo = (double)t + 0.25 * wv, hi = lo + 0.25;
+    // left walk: exactly u in [lo + H_8, hi + H_{K0-1}) (bounds on the bucket grid); right
+    // walk: u in (lo - H_{K0-1}, hi - H_8], one bucket wider (w = fl(10 - u)).  The two
+    // ranges are walked as one sequence of lanes (left walk first), so that a chunk's lanes
+    // are filled across the switch; keys side * npair + pair stay non-decreasing.
+    const int ja = src.sb[src.bk(lo + hk(8))], na = src.sb[src.bk(hi + hk(LANES_K0 - 1))] - ja;
+    const int jb = src.sb[src.bk(lo - hk(LANES_K0 - 1)) - 1];
+    const int ntot = na + (src.sb[src.bk(hi - hk(8)) + 1] - jb);
 #pragma unroll 1
-    for (int side = 0; side < 2; side++) {
-        int j0, j1;
-        if (side == 0) {  // exactly u in [lo + H_8, hi + H_{K0-1}): bounds on the bucket grid
-            j0 = src.sb[src.bk(lo + hk(8))];
-            j1 = src.sb[src.bk(hi + hk(LANES_K0 - 1))];
-        } else {          // u in (lo - H_{K0-1}, hi - H_8], one bucket wider (w = fl(10 - u))
-            j0 = src.sb[src.bk(lo - hk(LANES_K0 - 1)) - 1];
-            j1 = src.sb[src.bk(hi - hk(8)) + 1];
+    for (int c0 = 0; c0 < ntot; c0 += 64) {
+        const int m = c0 + lane;
+        const bool act = m < ntot;
+        const bool side = m >= na;  // this lane's walk: false left, true right
+        LineShape<R> p;
+        double w = 0.0;
+        if (act) {
+            p = src.line(side ? jb + (m - na) : ja + m);
+            double u;
+            src.centre(p, u);
+            w = side ? 10.0 - u : u;
         }
 #pragma unroll 1
-        for (int c0 = j0; c0 < j1; c0 += 64) {
-            const int j = c0 + lane;
-            const bool act = j < j1;
-            LineShape<R> p;
-            double w = 0.0;
+        for (int k = LANES_K0; k <= 9; k++) {
+            const double hq = 0.5 * hk(k);
+            const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
+            const int pw = npair / (BLOCK / 64), p0 = wv * pw;
+            int key = 2 * npair + lane;  // unique: lanes without a pair here form their own segments
+            R a0 = R(0), a1 = R(0), a2 = R(0), a2o = R(0), a3 = R(0), a4 = R(0);
             if (act) {
-                p = src.line(j);
-                double u;
-                src.centre(p, u);
-                w = side ? 10.0 - u : u;
+                const int nn = (int)floor(__builtin_ldexp(w, k));
+                const int gi = nn >> 1;
+                const int pi = side ? (5 << k) - gi - t * npair : gi - t * npair - 1;
+                if (pi >= p0 && pi < p0 + pw) {
+                    key = (side ? npair : 0) + pi;
+                    const bool five = nn & 1;
+                    const double q0 = (double)(4 * gi - 4);
+#pragma unroll 1
+                    for (int e = 0; e < (five ? 5 : 3); e++) {
+                        const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
+                        if (e == 0) a0 = v;
+                        else if (e == 1) a1 = v;
+                        else if (e == 2) {
+                            a2 = v;
+                            if (five) a2o = v;
+                        } else if (e == 3) a3 = v;
+                        else a4 = v;
+                    }
+                }
             }
-#pragma unroll 1
-            for (int k = LANES_K0; k <= 9; k++) {
-                const double hq = 0.5 * hk(k);
-                const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
-                const int pw = npair / (BLOCK / 64), p0 = wv * pw;
-                int key = npair + lane;  // unique: lanes without a pair here form their own segments
-                R a0 = R(0), a1 = R(0), a2 = R(0), a2o = R(0), a3 = R(0), a4 = R(0);
-                if (act) {
-                    const int nn = (int)floor(__builtin_ldexp(w, k));
-                    const int gi = nn >> 1;
-                    const int pi = side ? (5 << k) - gi - t * npair : gi - t * npair - 1;
-                    if (pi >= p0 && pi < p0 + pw) {
-                        key = pi;
-                        const bool five = nn & 1;
-                        const double q0 = (double)(4 * gi - 4);
-#pragma unroll 1
-                        for (int e = 0; e < (five ? 5 : 3); e++) {
-                            const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
-                            if (e == 0) a0 = v;
-                            else if (e == 1) a1 = v;
-                            else if (e == 2) {
-                                a2 = v;
-                                if (five) a2o = v;
-                            } else if (e == 3) a3 = v;
-                            else a4 = v;
-                        }
-                    }
-                }
-                // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
-                R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
-                R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
+            // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
+            R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
+            R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
 #pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const int kp = __shfl_up(key, off);
-                    const R u0 = __shfl_up(v0, off), u1 = __shfl_up(v1, off), u2 = __shfl_up(v2, off);
-                    const R u3 = __shfl_up(v3, off), u4 = __shfl_up(v4, off), u5 = __shfl_up(v5, off);
-                    if (lane >= off && kp == key) {
-                        v0 += u0; v1 += u1; v2 += u2; v3 += u3; v4 += u4; v5 += u5;
-                    }
+            for (int off = 1; off < 64; off <<= 1) {
+                const int kp = __shfl_up(key, off);
+                const R u0 = __shfl_up(v0, off), u1 = __shfl_up(v1, off), u2 = __shfl_up(v2, off);
+                const R u3 = __shfl_up(v3, off), u4 = __shfl_up(v4, off), u5 = __shfl_up(v5, off);
+                if (lane >= off && kp == key) {
+                    v0 += u0; v1 += u1; v2 += u2; v3 += u3; v4 += u4; v5 += u5;
                 }
-                const int kn = __shfl_down(key, 1);
-                if (key < npair && (lane == 63 || kn != key)) {
-                    const int cb = accb + 2 * key;
+            }
+            const int kn = __shfl_down(key, 1);
+            const bool last = key < 2 * npair && (lane == 63 || kn != key);
+            // a pair may end a segment in both walks of one chunk: left walk first, then
+            // right walk, so that no two lanes update one slot in the same instruction
+#pragma unroll
+            for (int sd = 0; sd < 2; sd++) {
+                if (last && (int)side == sd) {
+                    const int cb = accb + 2 * (key - (side ? npair : 0));
                     g[cb] += (double)v0;
                     g[cb + nk] += (double)v1;
                     g[cb + 2 * nk] += (double)v2;
