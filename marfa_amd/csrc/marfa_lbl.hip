@@ -312,11 +312,14 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         if (ctl.shape_id == 1 || (R)dmin * p.xs >= margin) {
             const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
             const R step = lw ? R(0.5) : R(-0.5);
+            // the Lorentz numerator (times chi) is carried as one FP64 factor, so a sample
+            // costs the reciprocal, its conversion and one FP64 multiply-add
             if (nochi) {
+                const double lorc = (double)p.lorc;
 #pragma unroll
                 for (int q = 0; q <= 20; q++) {
                     const R ax = q == 0 ? base : fma(step, (R)q, base);
-                    pos[q].add((double)(p.lorc * recip<R>(ax * ax + p.gl2)));
+                    pos[q].add((double)recip<R>(ax * ax + p.gl2) * lorc);
                 }
                 continue;
             }
@@ -325,13 +328,13 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
                 if (b == chi_band<R, CHI>((R)(dmin + 10.0))) {
                     R coef, rate;
                     chi_law<R, CHI>(b, coef, rate);
-                    double chi = (double)(coef * dexp<R>(-(rate * base)));
+                    double lchi = (double)p.lorc * (double)(coef * dexp<R>(-(rate * base)));
                     const double rho = rho_tab[2 * b + (lw ? 0 : 1)];
 #pragma unroll
                     for (int q = 0; q <= 20; q++) {
                         const R ax = q == 0 ? base : fma(step, (R)q, base);
-                        if (q > 0) chi *= rho;
-                        pos[q].add((double)(p.lorc * recip<R>(ax * ax + p.gl2)) * chi);
+                        if (q > 0) lchi *= rho;
+                        pos[q].add((double)recip<R>(ax * ax + p.gl2) * lchi);
                     }
                     continue;
                 }
