@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmarfa.so")
 HIP_SRC = [os.path.join(HERE, "csrc", "marfa_lbl.hip"), os.path.join(HERE, "csrc", "par_ingest.cpp")]
-HIP_DEPS = HIP_SRC + [os.path.join(HERE, "csrc", "lbl_device.h"), os.path.join(ROOT, "include", "marfa.h")]
+# every file the library is built from (sources, the kernel headers it includes, the C ABI)
+HIP_DEPS = HIP_SRC + sorted(os.path.join(HERE, "csrc", f) for f in os.listdir(os.path.join(HERE, "csrc"))
+                            if f.endswith((".h", ".inc"))) + [os.path.join(ROOT, "include", "marfa.h")]
 
 # packed FP32 instructions are switched off: the compiler emits ordinary VALU FP32 ops
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
