@@ -285,6 +285,89 @@ __device__ inline void gather_fine_cell(const Src &src, int i0, double v[4]) {
     v[3] = v3;
 }
 
+// Fine points of a staged tile in the FP32 form, one line per lane.  The four points of
+// grid-9 cell m (record-level index; points 4m+2 .. 4m+5) take the lines with u in
+// [(m-1) H9, (m+2) H9), i.e. a line with mu = floor(u / H9) serves cells mu-1, mu, mu+1,
+// under the same per-point tests as gather_fine_cell.  Wavefront wv owns the tile's cells
+// [128 wv, 128 wv + 128) -- lane l the cells 128 wv + l and 128 wv + 64 + l -- and walks
+// the staged lines of their windows 64 at a time: each lane evaluates its line's 12
+// points, a key-segmented lane scan (key = mu, non-decreasing along the sorted lines) sums
+// them per source cell, and the segment's last lane adds its three cells' sums to the
+// wavefront's chunk partials in LDS (one cell offset at a time, so no two lanes update one
+// slot in the same instruction); after each chunk the owning lanes add the partials to
+// their FP64 sums v[h][e] and clear them.  buf: 4 x 512 floats of free LDS.
+template <typename R>
+__device__ inline void fine_lanes(const CentreLds<R> &src, int t, float *buf, double v[2][4]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int M0 = 512 * t + 128 * wv;  // the wavefront's first cell (record level)
+    float *b = buf + 512 * wv;          // [128 cells][4 points]
+    for (int e = lane; e < 512; e += 64) b[e] = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) v[h][e] = 0.0;
+    const int j0 = src.sb[src.bk((double)(M0 - 1) * H9)], j1 = src.sb[src.bk((double)(M0 + 130) * H9)];
+#pragma unroll 1
+    for (int c0 = j0; c0 < j1; c0 += 64) {
+        const int j = c0 + lane;
+        int key = 1024 + lane;  // unique for lanes without a line
+        R a[12];
+#pragma unroll
+        for (int q = 0; q < 12; q++) a[q] = R(0);
+        if (j < j1) {
+            const LineShape<R> p = src.line(j);
+            double u;
+            src.centre(p, u);
+            const int mu = (int)floor(u * 512.0);
+            key = mu - M0 + 1;  // 0 .. 130
+#pragma unroll 1
+            for (int q = 0; q < 12; q++) {
+                const int m = mu - 1 + (q >> 2);
+                if (m < M0 || m >= M0 + 128) continue;
+                const int i0 = 2 + 4 * m, e = q & 3;
+                const double uhi = (double)(m + 2) * H9;
+                const double vhi = (double)(((NT - i0) >> 2) + 2) * H9;
+                const double xi = (double)(i0 + e - 1) * HF;
+                if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) continue;
+                const R x = shape_eval<R, true>(u - xi, p, src.ctl);
+#pragma unroll
+                for (int r = 0; r < 12; r++)
+                    if (r == q) a[r] = x;
+            }
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int kp = __shfl_up(key, off);
+            R w[12];
+#pragma unroll
+            for (int q = 0; q < 12; q++) w[q] = __shfl_up(a[q], off);
+            if (lane >= off && kp == key) {
+#pragma unroll
+                for (int q = 0; q < 12; q++) a[q] += w[q];
+            }
+        }
+        const int kn = __shfl_down(key, 1);
+        const bool last = key < 1024 && (lane == 63 || kn != key);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int cl = key - 2 + d;  // local cell of offset d (mu - 1 + d - M0)
+            if (last && cl >= 0 && cl < 128) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) b[4 * cl + e] += (float)a[4 * d + e];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int cl = lane + 64 * h;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                v[h][e] += (double)b[4 * cl + e];
+                b[4 * cl + e] = 0.0f;
+            }
+        }
+    }
+}
+
 // direct deposits of wing lines at the first / last fine points of the record
 template <typename R, typename Src>
 __device__ inline void gather_fine_wings(const Src &src, int i, Acc &acc) {
@@ -653,11 +736,19 @@ __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, 
     const int cb9 = lvl_acc_off(9);
     const double mul = (double)a.outmul[a.lev0 + lev] / sc;
     const int64_t ro = ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
+    // FP32 form, staged tile: lines per lane (grids 1..8's LDS is free after the cascade);
+    // otherwise each thread gathers the candidates of its cells
+    const bool lanes = sizeof(R) == 4 && staged;
+    double fv[2][4];
+    if (lanes) fine_lanes<R>(lsrc, t, (float *)g, fv);
     for (int h = 0; h < 2; h++) {
-        const int c = tid + h * BLOCK;
+        const int c = lanes ? 128 * (tid >> 6) + 64 * h + (tid & 63) : tid + h * BLOCK;
         const int i0 = 2048 * t + 2 + 4 * c;
         double v[4];
-        if (staged) gather_fine_cell<R>(lsrc, i0, v);
+        if (lanes) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) v[e] = fv[h][e];
+        } else if (staged) gather_fine_cell<R>(lsrc, i0, v);
         else gather_fine_cell<R>(gsrc, i0, v);
         if (i0 <= 5 || i0 + 3 >= NT - 3) {
 #pragma unroll 1
