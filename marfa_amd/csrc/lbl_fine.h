@@ -113,6 +113,9 @@ struct GlobalSrc {
     __device__ inline void fine_range(double ulo, double uhi, int &j0, int &j1) const {
         cand(f, ulo - 1e-9, uhi, j0, j1);
     }
+    // lines of the one-line-per-lane passes, u in [ulo, uhi): a bucket superset (the passes
+    // test centre membership and the deposit geometry per line)
+    __device__ inline void lane_range(double ulo, double uhi, int &j0, int &j1) const { cand(f, ulo, uhi, j0, j1); }
 };
 
 // Centre lines (class 2) of one tile's fine reach, staged in LDS in sorted order: u = nu* - org
@@ -162,6 +165,11 @@ struct CentreLds {
         }
     }
     __device__ inline void fine_range(double ulo, double uhi, int &j0, int &j1) const {
+        j0 = sb[bk(ulo)];
+        j1 = sb[bk(uhi)];
+    }
+    // u in [ulo, uhi) for bounds on the staged bucket grid
+    __device__ inline void lane_range(double ulo, double uhi, int &j0, int &j1) const {
         j0 = sb[bk(ulo)];
         j1 = sb[bk(uhi)];
     }
@@ -295,9 +303,10 @@ __device__ inline void gather_fine_cell(const Src &src, int i0, double v[4]) {
 // them per source cell, and the segment's last lane adds its three cells' sums to the
 // wavefront's chunk partials in LDS (one cell offset at a time, so no two lanes update one
 // slot in the same instruction); after each chunk the owning lanes add the partials to
-// their FP64 sums v[h][e] and clear them.  buf: 4 x 512 floats of free LDS.
-template <typename R>
-__device__ inline void fine_lanes(const CentreLds<R> &src, int t, float *buf, double v[2][4]) {
+// their FP64 sums v[h][e] and clear them.  buf: 4 x 512 floats of free LDS.  Src: the
+// tile's LDS stage, or the level's sorted records for a tile too dense to stage.
+template <typename R, typename Src>
+__device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2][4]) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int M0 = 512 * t + 128 * wv;  // the wavefront's first cell (record level)
     float *b = buf + 512 * wv;          // [128 cells][4 points]
@@ -306,7 +315,8 @@ __device__ inline void fine_lanes(const CentreLds<R> &src, int t, float *buf, do
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int e = 0; e < 4; e++) v[h][e] = 0.0;
-    const int j0 = src.sb[src.bk((double)(M0 - 1) * H9)], j1 = src.sb[src.bk((double)(M0 + 130) * H9)];
+    int j0, j1;
+    src.lane_range((double)(M0 - 1) * H9, (double)(M0 + 130) * H9, j0, j1);
 #pragma unroll 1
     for (int c0 = j0; c0 < j1; c0 += 64) {
         const int j = c0 + lane;
@@ -314,10 +324,10 @@ __device__ inline void fine_lanes(const CentreLds<R> &src, int t, float *buf, do
         R a[12];
 #pragma unroll
         for (int q = 0; q < 12; q++) a[q] = R(0);
-        if (j < j1) {
-            const LineShape<R> p = src.line(j);
-            double u;
-            src.centre(p, u);
+        LineShape<R> p;
+        double u = 0.0;
+        if (j < j1) p = src.line(j);
+        if (j < j1 && src.centre(p, u)) {
             const int mu = (int)floor(u * 512.0);
             key = mu - M0 + 1;  // 0 .. 130
 #pragma unroll 1
@@ -487,8 +497,8 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
 // first.
 constexpr int LANES_K0 = 6;
 
-template <typename R>
-__device__ inline void level_lanes(const CentreLds<R> &src, int t, double *g) {
+template <typename R, typename Src>
+__device__ inline void level_lanes(const Src &src, int t, double *g) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll 1
     for (int k = LANES_K0; k <= 9; k++) {
@@ -504,20 +514,21 @@ __device__ inline void level_lanes(const CentreLds<R> &src, int t, double *g) {
     // walk: u in (lo - H_{K0-1}, hi - H_8], one bucket wider (w = fl(10 - u)).  The two
     // ranges are walked as one sequence of lanes (left walk first), so that a chunk's lanes
     // are filled across the switch; keys side * npair + pair stay non-decreasing.
-    const int ja = src.sb[src.bk(lo + hk(8))], na = src.sb[src.bk(hi + hk(LANES_K0 - 1))] - ja;
-    const int jb = src.sb[src.bk(lo - hk(LANES_K0 - 1)) - 1];
-    const int ntot = na + (src.sb[src.bk(hi - hk(8)) + 1] - jb);
+    int ja, ja1, jb, jb1;
+    src.lane_range(lo + hk(8), hi + hk(LANES_K0 - 1), ja, ja1);
+    src.lane_range(lo - hk(LANES_K0 - 1) - 1.0 / BUCKETS_PER_CM, hi - hk(8) + 1.0 / BUCKETS_PER_CM, jb, jb1);
+    const int na = ja1 - ja, ntot = na + (jb1 - jb);
 #pragma unroll 1
     for (int c0 = 0; c0 < ntot; c0 += 64) {
         const int m = c0 + lane;
-        const bool act = m < ntot;
+        bool act = m < ntot;
         const bool side = m >= na;  // this lane's walk: false left, true right
         LineShape<R> p;
         double w = 0.0;
         if (act) {
             p = src.line(side ? jb + (m - na) : ja + m);
             double u;
-            src.centre(p, u);
+            act = src.centre(p, u);  // staged lines always; sorted records: centre lines only
             w = side ? 10.0 - u : u;
         }
 #pragma unroll 1
@@ -593,8 +604,11 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 #pragma unroll 1
     for (int k = 1; k <= 9; k++) {
-        if (k >= LANES_K0 && staged) {
-            if (k == LANES_K0) level_lanes<R>(lsrc, t, g);
+        if (k >= LANES_K0) {
+            if (k == LANES_K0) {
+                if (staged) level_lanes<R>(lsrc, t, g);
+                else level_lanes<R>(gsrc, t, g);
+            }
             if ((t == 0 || t == TILES - 1) && wv == (t == 0 ? 0 : BLOCK / 64 - 1)) {
                 // wing lines of the record's edge pair (not staged), by the wavefront that
                 // owns the pair, after its centre lines
@@ -705,7 +719,7 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t) {
 // One (record, tile) work item w by owner gathers: grid 1..9 sums into g, cascade,
 // fine points in registers, record write.  LDS: g[NACC], staging arrays of capacity cap.
 template <typename R>
-__device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, double *s_u, int32_t *s_j, int32_t *s_b,
+__device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, double *red, double *s_u, int32_t *s_j, int32_t *s_b,
                                  int *s_cnt, int cap) {
     const int r = w / TILES, t = w % TILES;
     const int lev = r / a.nrec, s = r % a.nrec;
@@ -736,11 +750,14 @@ __device__ inline void fine_tile(const KArgs &a, int w, double *g, double *red, 
     const int cb9 = lvl_acc_off(9);
     const double mul = (double)a.outmul[a.lev0 + lev] / sc;
     const int64_t ro = ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
-    // FP32 form, staged tile: lines per lane (grids 1..8's LDS is free after the cascade);
-    // otherwise each thread gathers the candidates of its cells
-    const bool lanes = sizeof(R) == 4 && staged;
+    // FP32 form: lines per lane (grids 1..8's LDS is free after the cascade); FP64 form:
+    // each thread gathers the candidates of its cells
+    const bool lanes = sizeof(R) == 4;
     double fv[2][4];
-    if (lanes) fine_lanes<R>(lsrc, t, (float *)g, fv);
+    if (lanes) {
+        if (staged) fine_lanes<R>(lsrc, t, (float *)g, fv);
+        else fine_lanes<R>(gsrc, t, (float *)g, fv);
+    }
     for (int h = 0; h < 2; h++) {
         const int c = lanes ? 128 * (tid >> 6) + 64 * h + (tid & 63) : tid + h * BLOCK;
         const int i0 = 2048 * t + 2 + 4 * c;
