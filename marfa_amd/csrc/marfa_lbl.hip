@@ -501,7 +501,7 @@ struct marfa_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     size_t budget = (size_t)24 << 30;  // bytes of per-(line, level) scratch per level batch
-    int variant = 1;                   // fine-kernel variant (launch_fine)
+    int variant = 4;                   // fine-kernel variant (launch_fine): stage 1024, plain order
     // inputs
     int64_t n = 0;
     std::vector<double> h_nu;
