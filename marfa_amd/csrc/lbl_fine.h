@@ -190,14 +190,22 @@ struct CentreLds {
 // the same pattern as a centre line's -- nodes q = 4-e from the edge, e = 0..2, plus
 // e = 3, 4 and the second cell's edge when d >= H_k (m == k) -- so they are summed in the
 // same loop.  X: left wing u - q hq, right wing (u - 10) + q' hq, as the reference forms them.
+// rng: the group's candidate range [rng[0], rng[1]) looked up beforehand (group_range of
+// the same group), or null
 template <typename R, bool RIGHT, typename Src, bool WINGS_ONLY = false>
-__device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, double s[6]) {
+__device__ inline void pair_walk(const Src &src, int k, int g, int sl, int L, double s[6],
+                                 const int *rng = nullptr) {
     const double Hk = hk(k), hq = 0.5 * Hk;
     const double wlo = (double)(2 * g) * Hk, whi = (double)(2 * g + 2) * Hk;
     const double q0 = (double)(4 * g - 4);
     const bool edge = g == (5 << k);  // walk group starting at the record's far edge
     int j0, j1;
-    src.template group_range<RIGHT>(wlo, whi, j0, j1);
+    if (rng) {
+        j0 = rng[0];
+        j1 = rng[1];
+    } else {
+        src.template group_range<RIGHT>(wlo, whi, j0, j1);
+    }
     // sums by position e = q - q0 (a2o: e = 2 of lines with n odd, the second cell's edge)
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a2o = 0.0, a3 = 0.0, a4 = 0.0;
     int j = j0 + sl;
@@ -602,10 +610,27 @@ template <typename R>
 __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds<R> &lsrc, bool staged, int t,
                                        double *g, double *red) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // Grids 1..3 read their lines from the sorted records in global memory.  Their 14
+    // candidate ranges (7 pairs x 2 walks) are looked up at once by 14 threads and parked in
+    // grid 9's slots, which nothing uses before the lane pass zeroes them: one wait for the
+    // bucket-index loads instead of one per grid and walk.
+    int *rng = (int *)(g + lvl_acc_off(9));
+    if (tid < 14) {
+        const int k = tid < 2 ? 1 : tid < 6 ? 2 : 3, e = tid - (k == 1 ? 0 : k == 2 ? 2 : 6);
+        const int npair = 1 << (k - 1), gl = t * npair + 1 + (e >> 1), gw = (e & 1) ? (5 << k) + 1 - gl : gl;
+        const double Hk = hk(k);
+        int j0, j1;
+        if (e & 1) gsrc.template group_range<true>((double)(2 * gw) * Hk, (double)(2 * gw + 2) * Hk, j0, j1);
+        else gsrc.template group_range<false>((double)(2 * gw) * Hk, (double)(2 * gw + 2) * Hk, j0, j1);
+        rng[2 * tid] = j0;
+        rng[2 * tid + 1] = j1;
+    }
+    __syncthreads();
 #pragma unroll 1
     for (int k = 1; k <= 9; k++) {
         if (k >= LANES_K0) {
             if (k == LANES_K0) {
+                __syncthreads();  // every wavefront is past grids 1..3 (their ranges in grid 9's slots)
                 if (staged) level_lanes<R>(lsrc, t, g);
                 else level_lanes<R>(gsrc, t, g);
             }
@@ -640,6 +665,10 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
         if (k >= 4 && staged && !edge) {
             pair_walk<R, false>(lsrc, k, gl, sl, L, s);
             pair_walk<R, true>(lsrc, k, gr, sl, L, s);
+        } else if (k <= 3) {
+            const int *r = rng + 2 * (2 * ((1 << (k - 1)) - 1) + 2 * pi);
+            pair_walk<R, false>(gsrc, k, gl, sl, L, s, r);
+            pair_walk<R, true>(gsrc, k, gr, sl, L, s, r + 2);
         } else {
             pair_walk<R, false>(gsrc, k, gl, sl, L, s);
             pair_walk<R, true>(gsrc, k, gr, sl, L, s);
