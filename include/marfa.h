@@ -116,7 +116,7 @@ int marfa_lbl_scheme(marfa_ctx *ctx, int64_t *line_idx, double cap_wv, double st
  * out_dev: device pointer (may be NULL) -- results stay in HBM;
  * out_host: host pointer (may be NULL) -- results copied back.
  * stream: hipStream_t to run on (NULL = the context's stream).  Synchronous unless
- * out_host is NULL and async != 0. */
+ * out_host is NULL and async != 0; an asynchronous call records no phase timings. */
 int marfa_pt_tables(marfa_ctx *ctx, const marfa_job *job, float *out_dev, float *out_host,
                     void *stream, int32_t async);
 
@@ -135,6 +135,22 @@ int marfa_pt_tables_f64(marfa_ctx *ctx, const marfa_job *job, double *out_dev, v
  * Not available for the Perrin chi factor (MARFA_E_ARG). */
 int marfa_pt_tables_adjoint(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
                             double *grad_lines, double *grad_levels);
+/* marfa_pt_tables_adjoint with the gradients in device memory (grad_lines [4][n], grad_levels
+ * [2][nlev], FP64), enqueued on stream without waiting: for callers whose tensors live on
+ * the GPU (the PyTorch operators in marfa_amd/csrc/torch_ops.cpp). */
+int marfa_pt_tables_adjoint_dev(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
+                                double *grad_lines, double *grad_levels);
+
+/* Replace the per-line S, gamma_air, n_T and delta of the context's line list (n values each,
+ * device pointers) by device-to-device copies on stream: the centres, and with them the
+ * windows, are unchanged, so nothing is re-allocated or re-checked on the host.  No
+ * counterpart in the reference, which reads these fields from the line records
+ * (src/LBL.f90:43); for differentiable callers that update them every step. */
+int marfa_update_line_params(marfa_ctx *ctx, const float *S, const float *gair, const float *nT,
+                             const float *delta, void *stream);
+
+/* The context's line count, level count and device (any pointer may be NULL). */
+int marfa_ctx_info(const marfa_ctx *ctx, int64_t *nlines, int32_t *nlev, int32_t *device);
 
 /* ---- batches of independent jobs in one hipGraph (no counterpart in the reference, which
  * runs one job per process) ----

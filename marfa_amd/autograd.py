@@ -1,65 +1,67 @@
-"""Differentiable PT tables: torch.autograd.Function over marfa_pt_tables and its adjoint.
+"""Differentiable PT tables: torch.autograd.Function over the marfa operators.
 
-Forward: the records of every level x subinterval on the GPU (marfa_pt_tables) for per-line
-S, gamma_air, n_T, delta and per-level T, P; every other input (centres, gamma_self, E'',
-isotopes, densities, TIPS) is held fixed.  Backward: marfa_pt_tables_adjoint with the
-incoming dL/d(records) (a CUDA tensor), which recomputes the grid sums and returns
-dL/d(line parameters) and dL/d(T, P) -- no grids are stored between the two calls.
-PyTorch only supplies device buffers and the stream; all arithmetic is in the HIP library.
+Forward: the records of every level x subinterval on the GPU (torch.ops.marfa.pt_tables) for
+per-line S, gamma_air, n_T, delta and per-level T, P; every other input (centres,
+gamma_self, E'', isotopes, densities, TIPS) is held fixed.  The line parameters reach the
+library as device tensors (device-to-device copies); the levels, a few values each, through
+the host.  Backward: torch.ops.marfa.pt_tables_adjoint with the incoming dL/d(records),
+which recomputes the grid sums and returns dL/d(line parameters) and dL/d(T, P) as device
+tensors -- no grids are stored between the two calls.  PyTorch only supplies device buffers
+and the stream; all arithmetic is in the HIP library.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from . import lbl, synth
+from . import lbl, ops
 
 
 class PTTableModel:
-    """A line list and atmosphere whose S, gamma_air, n_T, delta and T, P can vary."""
+    """A line list and atmosphere whose S, gamma_air, n_T, delta and T, P can vary.  The
+    model holds one parameter set at a time; `version` changes with every set_params, so a
+    backward pass can tell whether the parameters of its forward pass are still loaded."""
 
     def __init__(self, job, tips, precision=lbl.FP32K, device: int = 0):
         self.job = job
+        self.device = torch.device("cuda", device)
         self.ctx = lbl.LBLContext(device)
         self.ctx.set_tips(tips)
         self.spec = lbl.JobSpec.from_job(job, precision)
-        self.lines = job.lines
-        self.atm = job.atm
         self.ctx.set_lines(job.lines)
         self.ctx.set_levels(job.atm)
         lo, _, _, _ = self.ctx.windows(self.spec)
         self.shape = (len(job.atm), lo.shape[0], lbl.NT)
-        self.last_grads = None
+        f = lambda x: torch.tensor(np.asarray(x, np.float32), device=self.device)
+        L, A = job.lines, job.atm
+        self._params = (f(L.S), f(L.gair), f(L.nT), f(L.delta), f(A.T), f(A.P))
+        self._rho = f(A.rho)
+        self.version = 0
+        ops.load()
 
-    def params(self, device="cuda"):
-        """The current parameters as float32 tensors (the reference's FP32 inputs)."""
-        L, A = self.lines, self.atm
-        f = lambda x: torch.tensor(np.asarray(x, np.float32), device=device)
-        return f(L.S), f(L.gair), f(L.nT), f(L.delta), f(A.T), f(A.P)
+    def params(self):
+        """The current parameters as float32 device tensors (copies)."""
+        return tuple(p.clone() for p in self._params)
 
     def set_params(self, S, gair, nT, delta, T, P):
-        host = lambda t: t.detach().to("cpu", torch.float32).numpy()
-        L = self.lines
-        self.lines = synth.Lines(nu=L.nu, S=host(S), gair=host(gair), gself=L.gself, elow=L.elow,
-                                 nT=host(nT), delta=host(delta), molIso=L.molIso)
-        self.atm = synth.Atmosphere(P=host(P), T=host(T), rho=self.atm.rho, h=self.atm.h)
-        self.ctx.set_lines(self.lines)
-        self.ctx.set_levels(self.atm)
+        f = lambda t: t.detach().to(self.device, torch.float32).contiguous()
+        new = tuple(f(t) for t in (S, gair, nT, delta, T, P))
+        ops.set_line_params(self.ctx, *new[:4])
+        if not (torch.equal(new[4], self._params[4]) and torch.equal(new[5], self._params[5])):
+            ops.set_levels(self.ctx, new[5], new[4], self._rho)
+        self._params = tuple(t.clone() for t in new)
+        self.version += 1
 
     def records(self):
-        out = torch.empty(self.shape, dtype=torch.float32, device="cuda")
-        self.ctx.pt_tables_device(self.spec, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
-        return out
+        """Records [nlev, nrec, 20481] (float32, device) of the current parameters."""
+        return ops.pt_tables(self.ctx, self.spec)
 
     def records_f64(self):
         return self.ctx.pt_tables_f64_torch(self.spec)
 
     def adjoint(self, grad_out):
-        g = grad_out.detach().to(torch.float32).contiguous()
-        torch.cuda.current_stream().synchronize()
-        gl, gv = self.ctx.pt_tables_adjoint(self.spec, g.data_ptr(), torch.cuda.current_stream().cuda_stream)
-        self.last_grads = (gl, gv)
-        return gl, gv
+        """(dL/d(S, gamma_air, n_T, delta) [4, n], dL/d(T, P) [2, nlev]): float64 device tensors."""
+        return ops.pt_tables_adjoint(self.ctx, self.spec, grad_out.detach())
 
     def close(self):
         self.ctx.close()
@@ -72,15 +74,19 @@ class PTTables(torch.autograd.Function):
     def forward(ctx, S, gair, nT, delta, T, P, model):
         model.set_params(S, gair, nT, delta, T, P)
         ctx.model = model
-        ctx.meta = [(t.dtype, t.device) for t in (S, gair, nT, delta, T, P)]
+        ctx.version = model.version
+        ctx.save_for_backward(S, gair, nT, delta, T, P)
         return model.records()
 
     @staticmethod
     def backward(ctx, grad_out):
+        saved = ctx.saved_tensors
+        if ctx.model.version != ctx.version:
+            # the model was applied again since this node's forward: reload its parameters
+            ctx.model.set_params(*saved)
         gl, gv = ctx.model.adjoint(grad_out)
-        outs = [gl[0], gl[1], gl[2], gl[3], gv[0], gv[1]]
-        grads = tuple(torch.as_tensor(g, dtype=dt, device=dev) for g, (dt, dev) in zip(outs, ctx.meta))
-        return grads + (None,)
+        outs = (gl[0], gl[1], gl[2], gl[3], gv[0], gv[1])
+        return tuple(g.to(device=t.device, dtype=t.dtype) for g, t in zip(outs, saved)) + (None,)
 
 
 def spectral_loss(records, target, weight=1.0):

@@ -41,6 +41,30 @@ def build_hip(verbose: bool = False) -> str:
     return LIB
 
 
+TORCH_OPS_SRC = os.path.join(HERE, "csrc", "torch_ops.cpp")
+TORCH_OPS_LIB = os.path.join(HERE, "libmarfa_torch.so")
+
+
+def build_torch_ops(verbose: bool = False) -> str:
+    """torch.ops.marfa.* (csrc/torch_ops.cpp): host code over the C ABI, compiled by hipcc
+    against the installed PyTorch-ROCm and linked to libmarfa.so beside it."""
+    import torch
+    from torch.utils import cpp_extension as ce
+    tlib = ce.library_paths()[0]
+    cmd = [hipcc(), "-O2", "-std=c++17", "-fPIC", "-shared", "-DUSE_ROCM",
+           f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}", "-Wno-unused-result"]
+    cmd += ["-I" + d for d in ce.include_paths()]
+    cmd += ["-o", TORCH_OPS_LIB + ".tmp", TORCH_OPS_SRC, "-L" + tlib, "-ltorch", "-ltorch_cpu", "-lc10",
+            "-lc10_hip", "-ltorch_hip", "-L" + HERE, "-lmarfa", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + tlib]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc (torch operators) failed:\n" + r.stderr[-4000:])
+    os.replace(TORCH_OPS_LIB + ".tmp", TORCH_OPS_LIB)
+    return TORCH_OPS_LIB
+
+
 def build_fortran_host(verbose: bool = False) -> str | None:
     """amdflang host programs marfa_amd/fortran/marfa_gpu (drop-in for app/main.f90) and
     marfa_amd/fortran/marfa_par (drop-in for app/processParFile.f90)."""
@@ -68,4 +92,5 @@ def build_fortran_host(verbose: bool = False) -> str | None:
 
 def build_all(verbose: bool = False):
     build_hip(verbose)
+    build_torch_ops(verbose)
     build_fortran_host(verbose)

@@ -229,3 +229,26 @@ __global__ __launch_bounds__(BLOCK) void adjoint_reduce_kernel(const double *gli
     gacc[j * 4 + 2] = s2;
     gacc[j * 4 + 3] = s3;
 }
+
+// per-level dL/dT and dL/dP of a level batch: the block partials of adjoint_line_kernel
+// summed in block order; out[l] (T) and out[nlev + l] (P) for the batch's levels from l0
+__global__ __launch_bounds__(BLOCK) void adjoint_level_kernel(const double *glev, int nb, int nblk, int nlev, int l0,
+                                                             double *out) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= 2 * nb) return;
+    const int q = t >> 1, c = t & 1;
+    double x = 0.0;
+    for (int b2 = 0; b2 < nblk; b2++) x += glev[((int64_t)q * nblk + b2) * 2 + c];
+    out[(int64_t)c * nlev + l0 + q] = x;
+}
+
+// per-line gradients [nl][4] (lines lmin .. lmin + nl - 1) into the caller's layout
+// out[q * n + i] (q = S, gamma_air, n_T, delta), zero for the lines outside every window
+__global__ __launch_bounds__(BLOCK) void adjoint_scatter_kernel(const double *gacc, int64_t lmin, int64_t nl, int64_t n,
+                                                               double *out) {
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const bool in = i >= lmin && i < lmin + nl;
+#pragma unroll
+        for (int q = 0; q < 4; q++) out[q * n + i] = in ? gacc[(i - lmin) * 4 + q] : 0.0;
+    }
+}

@@ -31,6 +31,20 @@ __device__ inline int bucket_of(double nus, double base, int NB) {
     return (int)x;
 }
 
+// Key of a lane for a key-segmented lane scan.  Lanes with a value (has) keep their key;
+// a lane without one takes the key of the nearest lane before it that has one (its own
+// values are zero), or -1 before the first.  Along the lanes with values the keys are
+// non-decreasing, but lanes without one can sit between two lanes of one key (a line
+// outside the record's window among the sorted candidates); inheriting the key keeps every
+// key's lanes contiguous, so the scan's segment ends hold each key's sum exactly once.
+__device__ inline int scan_key(bool has, int key) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t before = __ballot(has) & ((1ull << lane) - 1ull);
+    const int from = before ? 63 - __clzll(before) : lane;
+    const int k = __shfl(key, from);
+    return has ? key : (before ? k : -1);
+}
+
 template <typename R>
 struct FineCtx {
     const uint64_t *skey;
@@ -328,14 +342,15 @@ __device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2]
 #pragma unroll 1
     for (int c0 = j0; c0 < j1; c0 += 64) {
         const int j = c0 + lane;
-        int key = 1024 + lane;  // unique for lanes without a line
+        int key = -1;  // lanes without a centre line: see scan_key
         R a[12];
 #pragma unroll
         for (int q = 0; q < 12; q++) a[q] = R(0);
         LineShape<R> p;
         double u = 0.0;
         if (j < j1) p = src.line(j);
-        if (j < j1 && src.centre(p, u)) {
+        const bool has = j < j1 && src.centre(p, u);
+        if (has) {
             const int mu = (int)floor(u * 512.0);
             key = mu - M0 + 1;  // 0 .. 130
 #pragma unroll 1
@@ -353,6 +368,7 @@ __device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2]
                     if (r == q) a[r] = x;
             }
         }
+        key = scan_key(has, key);
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const int kp = __shfl_up(key, off);
@@ -365,7 +381,7 @@ __device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2]
             }
         }
         const int kn = __shfl_down(key, 1);
-        const bool last = key < 1024 && (lane == 63 || kn != key);
+        const bool last = key >= 0 && (lane == 63 || kn != key);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const int cl = key - 2 + d;  // local cell of offset d (mu - 1 + d - M0)
@@ -498,7 +514,7 @@ __device__ inline bool stage_tile(const FineCtx<R> &f, int t, double *s_u, int32
 // once and, grid after grid, evaluates its 3 or 5 positions when the line's pair (the same
 // floor as the deposit geometry) is one of the wavefront's; a key-segmented inclusive scan
 // across the lanes (keys = pair index, non-decreasing along the sorted lines in both walks;
-// lanes with no pair on that grid have unique keys) sums every pair's values, and the last
+// lanes with no pair on that grid inherit the key before them, scan_key) sums every pair's values, and the last
 // lane of each segment adds them to the pair's slots in LDS.  Pairs belong to one
 // wavefront and its chunks run in order, so every sum has a fixed order.  Centre lines
 // only (wing lines are added by the edge pair's walk); the wavefront's slots are zeroed
@@ -544,7 +560,7 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
             const double hq = 0.5 * hk(k);
             const int npair = 1 << (k - 1), nk = 1 << k, accb = lvl_acc_off(k);
             const int pw = npair / (BLOCK / 64), p0 = wv * pw;
-            int key = 2 * npair + lane;  // unique: lanes without a pair here form their own segments
+            int key = -1;  // lanes without a pair on this grid: see scan_key
             R a0 = R(0), a1 = R(0), a2 = R(0), a2o = R(0), a3 = R(0), a4 = R(0);
             if (act) {
                 const int nn = (int)floor(__builtin_ldexp(w, k));
@@ -570,6 +586,7 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
             // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
             R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
             R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
+            key = scan_key(key >= 0, key);
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
                 const int kp = __shfl_up(key, off);
@@ -580,13 +597,14 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
                 }
             }
             const int kn = __shfl_down(key, 1);
-            const bool last = key < 2 * npair && (lane == 63 || kn != key);
+            const bool last = key >= 0 && (lane == 63 || kn != key);
+            const bool kside = key >= npair;  // the walk of the segment (a lane may inherit it)
             // a pair may end a segment in both walks of one chunk: left walk first, then
             // right walk, so that no two lanes update one slot in the same instruction
 #pragma unroll
             for (int sd = 0; sd < 2; sd++) {
-                if (last && (int)side == sd) {
-                    const int cb = accb + 2 * (key - (side ? npair : 0));
+                if (last && (int)kside == sd) {
+                    const int cb = accb + 2 * (key - (kside ? npair : 0));
                     g[cb] += (double)v0;
                     g[cb + nk] += (double)v1;
                     g[cb + 2 * nk] += (double)v2;

@@ -525,6 +525,7 @@ struct marfa_ctx {
     Scratch own;
     Scratch *s = &own;
     Buf lvl, win, out, counts;  // level and window arrays of the last run, records, counters
+    Buf adjout;                 // device gradients of marfa_pt_tables_adjoint before their copy
     hipEvent_t ev[5] = {};
     float timings[5] = {0, 0, 0, 0, 0};
     double *dbg_grids = nullptr;
@@ -609,20 +610,35 @@ struct LevelSet {  // host-side level state for one run
 };
 
 // Run levels x records.  Windows are level-independent; out is [nlev][nrec][NT].
+// Fixed device memory from which run() carves its scratch instead of allocating it
+// (for callers that must not allocate): a bump allocator of 256-byte aligned pieces.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    static size_t round(size_t b) { return (b + 255) & ~(size_t)255; }
+    void *take(size_t bytes) {
+        if (used + round(bytes) > size) return nullptr;
+        void *p = base + used;
+        used += round(bytes);
+        return p;
+    }
+};
+
 // Adjoint request of run(): gout = dL/d(record point) in the records' layout (device);
-// the per-line gradients accumulate in c->s->gacc ([nl][4] from line lmin), the per-level
-// T, P gradients in tp (host, [2][nlev]).
+// the per-line gradients accumulate in c->s->gacc ([nl][4] from line lmin) and land in
+// glines (device, [4][n]), the per-level T, P gradients in glevels (device, [2][nlev]).
 struct AdjReq {
     const float *gout = nullptr;
+    double *glines = nullptr, *glevels = nullptr;
     int64_t lmin = 0, nl = 0;
-    std::vector<double> *tp = nullptr;
 };
 
 // capture: the kernels only, for recording into a hipGraph -- the level and window arrays
 // must already be on the device (an earlier run of the same job on the same context) and
 // every buffer already large enough, so that no allocation, copy or synchronisation occurs.
 int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
-        hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr, bool capture = false) {
+        hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr, bool capture = false,
+        Arena *arena = nullptr) {
     if (capture) timed = false;
     const int nlev = (int)L.P.size(), nrec = W.nrec;
     if (nrec == 0 || nlev == 0) return 0;
@@ -630,8 +646,29 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         if (!(T >= 20.f && T < 1018.f)) return set_err(c, MARFA_E_TEMPERATURE, "temperature outside the TIPS table");
     const int64_t lmin = *std::min_element(W.lo.begin(), W.lo.end());
     const int64_t lmax = *std::max_element(W.hi.begin(), W.hi.end());
-    const int64_t nl = std::max<int64_t>(lmax - lmin, 1);
     if (lmin < 0 || lmax > c->n) return set_err(c, MARFA_E_WINDOW, "window outside the line list");
+    if (lmax <= lmin) {
+        // every window is empty (the range lies beyond the line list): the records are zero,
+        // cleared by a kernel (no memset, so a capture holds it too); no line is read
+        const int64_t words = (int64_t)nlev * nrec * NT * (out64 ? 2 : 1);
+        int32_t *dst = out64 ? (int32_t *)out64 : (int32_t *)out;
+        if (dst) {
+            hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((words + BLOCK - 1) / BLOCK, 65536)),
+                               dim3(BLOCK), 0, st, dst, words, 0);
+            HIPCHK(hipGetLastError());
+        }
+        if (adj) {
+            adj->lmin = lmin;
+            adj->nl = 0;
+            hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((8 * c->n + BLOCK - 1) / BLOCK, 65536)),
+                               dim3(BLOCK), 0, st, (int32_t *)adj->glines, 8 * c->n, 0);
+            hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(BLOCK), 0, st, (int32_t *)adj->glevels,
+                               (int64_t)4 * nlev, 0);
+            HIPCHK(hipGetLastError());
+        }
+        return 0;
+    }
+    const int64_t nl = lmax - lmin;
 
     // bucket index of shifted centres: 1/512 cm-1 buckets over every record's reach
     const double bbase = W.org.front() - (double)job.cutoff - 4.0;
@@ -639,15 +676,44 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     const int NB = (int)std::ceil((btop - bbase) * 512.0) + 1;
     const size_t shp_bytes = job.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
 
-    // level batches so that per-(line, level) arrays stay within a memory budget
+    const int nsblk = (NB + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    const unsigned nblk = (unsigned)((nl + BLOCK - 1) / BLOCK);
+
+    // level batches so that per-(line, level) arrays stay within a memory budget -- or, with
+    // an arena (the allocation-free entry point), within what the arena holds
     size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(2 * NB + 1) * 4;
     if (adj) per_level += (size_t)nrec * (30690 + NT) * 4 + (size_t)nl * 4 * 8;
     const size_t budget = c->budget;
     int lb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nlev, budget / std::max<size_t>(per_level, 1)));
+    const size_t lvl_bytes = (size_t)nlev * (4 * 4 + 2 * 8) + 64, win_bytes = (size_t)nrec * 24 + 64;
+    if (arena) {
+        // the level and window arrays, then per level of a batch every scratch array, each
+        // piece rounded up to the arena's 256-byte alignment (13 pieces per batch)
+        const size_t fixed = Arena::round(lvl_bytes) + Arena::round(win_bytes) + 13 * 256;
+        const size_t one = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)nrec * NCOARSE * 8 +
+                           (size_t)(2 * NB + 1) * 4 + (size_t)nsblk * 4;
+        if (adj || arena->size < fixed + one)
+            return set_err(c, MARFA_E_ARG, "the job needs " + std::to_string(fixed + one) +
+                                               " bytes of scratch for one level; the fixed scratch holds " +
+                                               std::to_string(arena->size));
+        lb = (int)std::min<size_t>((size_t)nlev, (arena->size - fixed) / one);
+    }
+    // scratch arrays: the context's buffers (grown on demand) or carved from the arena
+    auto get = [&](marfa_ctx::Buf &b, size_t bytes) -> void * {
+        if (arena) return arena->take(bytes);
+        return ensure(c, b, bytes) ? nullptr : b.p;
+    };
+#define MARFA_GET(var, buf, bytes)                                                                   \
+    void *var = get(buf, bytes);                                                                     \
+    if (!var) return arena ? set_err(c, MARFA_E_ARG, "fixed scratch exhausted") : MARFA_E_HIP;
 
     // level arrays [nlev]: P, T, outmul (float), pself, sscale (double), max exponent (int)
-    if (!capture && ensure(c, c->lvl, (size_t)nlev * (4 * 4 + 2 * 8) + 64)) return MARFA_E_HIP;
-    float *dP = (float *)c->lvl.p, *dT = dP + nlev, *dmul = dT + nlev;
+    void *p_lvl = c->lvl.p;
+    if (!capture) {
+        p_lvl = get(c->lvl, lvl_bytes);
+        if (!p_lvl) return MARFA_E_HIP;
+    }
+    float *dP = (float *)p_lvl, *dT = dP + nlev, *dmul = dT + nlev;
     double *dps = (double *)(((uintptr_t)(dmul + nlev) + 15) & ~(uintptr_t)15), *dsc = dps + nlev;
     int *demax = (int *)(dsc + nlev);
     if (!capture) {
@@ -657,8 +723,12 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
     }
     // windows
-    if (!capture && ensure(c, c->win, (size_t)nrec * 24 + 64)) return MARFA_E_HIP;
-    int64_t *dlo = (int64_t *)c->win.p, *dhi = dlo + nrec;
+    void *p_win = c->win.p;
+    if (!capture) {
+        p_win = get(c->win, win_bytes);
+        if (!p_win) return MARFA_E_HIP;
+    }
+    int64_t *dlo = (int64_t *)p_win, *dhi = dlo + nrec;
     double *dorg = (double *)(dhi + nrec);
     if (!capture) {
         HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
@@ -667,27 +737,31 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     }
 
     const size_t arr = (size_t)lb * nl;
-    for (int q = 0; q < 4; q++)
-        if (ensure(c, c->s->params[q], arr * 8)) return MARFA_E_HIP;
-    for (int q = 0; q < 2; q++) {
-        if (ensure(c, c->s->keys[q], arr * 8)) return MARFA_E_HIP;
-        if (ensure(c, c->s->idx[q], arr * 4)) return MARFA_E_HIP;
-    }
-    if (ensure(c, c->s->coarse, (size_t)lb * nrec * NCOARSE * 8)) return MARFA_E_HIP;
-    if (ensure(c, c->s->shapes, arr * shp_bytes)) return MARFA_E_HIP;
-    if (ensure(c, c->s->buckets, (size_t)lb * (NB + 1) * 4)) return MARFA_E_HIP;
-    const int nsblk = (NB + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    if (ensure(c, c->s->hist, (size_t)lb * NB * 4) || ensure(c, c->s->spart, (size_t)lb * nsblk * 4))
-        return MARFA_E_HIP;
-    const unsigned nblk = (unsigned)((nl + BLOCK - 1) / BLOCK);
+    MARFA_GET(p_par0, c->s->params[0], arr * 8)
+    MARFA_GET(p_par1, c->s->params[1], arr * 8)
+    MARFA_GET(p_par2, c->s->params[2], arr * 8)
+    MARFA_GET(p_par3, c->s->params[3], arr * 8)
+    MARFA_GET(p_key0, c->s->keys[0], arr * 8)
+    MARFA_GET(p_key1, c->s->keys[1], arr * 8)
+    MARFA_GET(p_idx0, c->s->idx[0], arr * 4)
+    MARFA_GET(p_idx1, c->s->idx[1], arr * 4)
+    MARFA_GET(p_coarse, c->s->coarse, (size_t)lb * nrec * NCOARSE * 8)
+    MARFA_GET(p_shapes, c->s->shapes, arr * shp_bytes)
+    MARFA_GET(p_buckets, c->s->buckets, (size_t)lb * (NB + 1) * 4)
+    MARFA_GET(p_hist, c->s->hist, (size_t)lb * NB * 4)
+    MARFA_GET(p_spart, c->s->spart, (size_t)lb * nsblk * 4)
+    void *p_gbar = nullptr, *p_gline = nullptr, *p_glev = nullptr, *p_gacc = nullptr;
     if (adj) {
-        if (ensure(c, c->s->gbar, (size_t)lb * nrec * (30690 + NT) * 4) || ensure(c, c->s->gline, arr * 4 * 8) ||
-            ensure(c, c->s->glev, (size_t)lb * nblk * 2 * 8) || ensure(c, c->s->gacc, (size_t)nl * 4 * 8))
+        if (!(p_gbar = get(c->s->gbar, (size_t)lb * nrec * (30690 + NT) * 4)) ||
+            !(p_gline = get(c->s->gline, arr * 4 * 8)) || !(p_glev = get(c->s->glev, (size_t)lb * nblk * 2 * 8)) ||
+            !(p_gacc = get(c->s->gacc, (size_t)nl * 4 * 8)))
             return MARFA_E_HIP;
-        HIPCHK(hipMemsetAsync(c->s->gacc.p, 0, (size_t)nl * 4 * 8, st));
+        // cleared by a kernel, not a memset, so that the sequence can be captured
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((8 * nl + BLOCK - 1) / BLOCK, 65536)),
+                           dim3(BLOCK), 0, st, (int32_t *)p_gacc, 8 * nl, 0);
+        HIPCHK(hipGetLastError());
         adj->lmin = lmin;
         adj->nl = nl;
-        adj->tp->assign(2 * (size_t)nlev, 0.0);
     }
 
     // FP32 form: per-level power-of-two sample scale from the maximum exponent of S(T);
@@ -715,12 +789,12 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         sa.T = dT + l0;
         sa.pself = dps + l0;
         sa.tips = c->d_tips;
-        sa.nus = (double *)c->s->params[0].p;
-        sa.gD = (double *)c->s->params[1].p;
-        sa.gL = (double *)c->s->params[2].p;
-        sa.ST = (double *)c->s->params[3].p;
-        sa.keys = (uint64_t *)c->s->keys[0].p;
-        sa.idx = (int32_t *)c->s->idx[0].p;
+        sa.nus = (double *)p_par0;
+        sa.gD = (double *)p_par1;
+        sa.gL = (double *)p_par2;
+        sa.ST = (double *)p_par3;
+        sa.keys = (uint64_t *)p_key0;
+        sa.idx = (int32_t *)p_idx0;
         sa.emax = job.precision == MARFA_FP32K ? demax + l0 : nullptr;
         dim3 gs((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
         hipLaunchKernelGGL(spectroscopy_kernel, gs, dim3(BLOCK), 0, st, sa);
@@ -731,10 +805,10 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         {
             // order by shifted centre: bucket counts, their scan (the bucket index), scatter,
             // in-bucket order
-            int32_t *hist = (int32_t *)c->s->hist.p, *bst = (int32_t *)c->s->buckets.p;
-            int32_t *part = (int32_t *)c->s->spart.p;
-            const uint64_t *k0 = (const uint64_t *)c->s->keys[0].p;
-            const int32_t *i0 = (const int32_t *)c->s->idx[0].p;
+            int32_t *hist = (int32_t *)p_hist, *bst = (int32_t *)p_buckets;
+            int32_t *part = (int32_t *)p_spart;
+            const uint64_t *k0 = (const uint64_t *)p_key0;
+            const int32_t *i0 = (const int32_t *)p_idx0;
             const int64_t nh = (int64_t)nb * NB;
             hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((nh + BLOCK - 1) / BLOCK, 65536)),
                                dim3(BLOCK), 0, st, hist, nh, 0);
@@ -746,21 +820,21 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             hipLaunchKernelGGL(scan_final_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB,
                                nsblk, (const int32_t *)part, bst);
             hipLaunchKernelGGL(bucket_scatter_kernel, gl, dim3(BLOCK), 0, st, k0, i0, nl, bbase, NB,
-                               (const int32_t *)bst, hist, (uint64_t *)c->s->keys[1].p, (int32_t *)c->s->idx[1].p);
+                               (const int32_t *)bst, hist, (uint64_t *)p_key1, (int32_t *)p_idx1);
             // final order into keys[0] / idx[0] (the unsorted copies are not read again) and
             // the line records
             if (job.precision == MARFA_FP32K)
-                hipLaunchKernelGGL(order_shape_kernel<float>, gl, dim3(BLOCK), 0, st, (const uint64_t *)c->s->keys[1].p,
-                                   (const int32_t *)c->s->idx[1].p, nl, bbase, NB, (const int32_t *)bst, lmin,
+                hipLaunchKernelGGL(order_shape_kernel<float>, gl, dim3(BLOCK), 0, st, (const uint64_t *)p_key1,
+                                   (const int32_t *)p_idx1, nl, bbase, NB, (const int32_t *)bst, lmin,
                                    (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
-                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)c->s->keys[0].p,
-                                   (int32_t *)c->s->idx[0].p, (LineShape<float> *)c->s->shapes.p);
+                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)p_key0,
+                                   (int32_t *)p_idx0, (LineShape<float> *)p_shapes);
             else
-                hipLaunchKernelGGL(order_shape_kernel<double>, gl, dim3(BLOCK), 0, st, (const uint64_t *)c->s->keys[1].p,
-                                   (const int32_t *)c->s->idx[1].p, nl, bbase, NB, (const int32_t *)bst, lmin,
+                hipLaunchKernelGGL(order_shape_kernel<double>, gl, dim3(BLOCK), 0, st, (const uint64_t *)p_key1,
+                                   (const int32_t *)p_idx1, nl, bbase, NB, (const int32_t *)bst, lmin,
                                    (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
-                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)c->s->keys[0].p,
-                                   (int32_t *)c->s->idx[0].p, (LineShape<double> *)c->s->shapes.p);
+                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)p_key0,
+                                   (int32_t *)p_idx0, (LineShape<double> *)p_shapes);
             HIPCHK(hipGetLastError());
         }
         if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
@@ -774,8 +848,8 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.gD = sa.gD;
         ka.gL = sa.gL;
         ka.S = sa.ST;
-        ka.skey = (const uint64_t *)c->s->keys[0].p;
-        ka.sidx = (const int32_t *)c->s->idx[0].p;
+        ka.skey = (const uint64_t *)p_key0;
+        ka.sidx = (const int32_t *)p_idx0;
         ka.lo = dlo;
         ka.hi = dhi;
         ka.org = dorg;
@@ -786,12 +860,12 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.shape_id = job.shape_id;
         ka.chi_id = job.chi_id;
         ka.mol = job.mol_code;
-        ka.coarse = (double *)c->s->coarse.p;
+        ka.coarse = (double *)p_coarse;
         ka.out = out;
         ka.out64 = out64;
         ka.grids = c->dbg_grids;
-        ka.shp = c->s->shapes.p;
-        ka.bst = (const int32_t *)c->s->buckets.p;
+        ka.shp = p_shapes;
+        ka.bst = (const int32_t *)p_buckets;
         ka.bbase = bbase;
         ka.NB = NB;
         const int nitems = nb * nrec;
@@ -819,7 +893,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
         if (adj) {
             // adjoint: dL/dnode of every record, then the line pass and its reductions
-            float *gbar = (float *)c->s->gbar.p;
+            float *gbar = (float *)p_gbar;
             if (job.precision == MARFA_FP32K)
                 hipLaunchKernelGGL(adjoint_tile_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
             else
@@ -838,24 +912,19 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             ab.P = dP;
             ab.pself = dps;
             ab.gbar = gbar;
-            ab.gline = (double *)c->s->gline.p;
-            ab.glev = (double *)c->s->glev.p;
+            ab.gline = (double *)p_gline;
+            ab.glev = (double *)p_glev;
             if (job.precision == MARFA_FP32K)
                 hipLaunchKernelGGL(adjoint_line_kernel<float>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
             else
                 hipLaunchKernelGGL(adjoint_line_kernel<double>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)c->s->gline.p, nb,
-                               nl, (double *)c->s->gacc.p);
+            hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)p_gline, nb,
+                               nl, (double *)p_gacc);
             HIPCHK(hipGetLastError());
-            std::vector<double> hl((size_t)nb * nblk * 2);
-            HIPCHK(hipMemcpyAsync(hl.data(), c->s->glev.p, hl.size() * 8, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (int q = 0; q < nb; q++)
-                for (unsigned b2 = 0; b2 < nblk; b2++) {
-                    (*adj->tp)[l0 + q] += hl[((size_t)q * nblk + b2) * 2];
-                    (*adj->tp)[nlev + l0 + q] += hl[((size_t)q * nblk + b2) * 2 + 1];
-                }
+            hipLaunchKernelGGL(adjoint_level_kernel, dim3((2 * nb + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st,
+                               (const double *)p_glev, nb, (int)nblk, nlev, l0, adj->glevels);
+            HIPCHK(hipGetLastError());
         } else if (job.precision == MARFA_FP32K) {
             launch_fine<float>(c->variant, dim3(gfine), st, ka);
         } else {
@@ -872,12 +941,18 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             }
         }
     }
+    if (adj) {
+        hipLaunchKernelGGL(adjoint_scatter_kernel, dim3((unsigned)std::min<int64_t>((c->n + BLOCK - 1) / BLOCK, 65536)),
+                           dim3(BLOCK), 0, st, (const double *)p_gacc, lmin, nl, c->n, adj->glines);
+        HIPCHK(hipGetLastError());
+    }
     if (timed) {
         tms[4] = tms[0] + tms[1] + tms[2] + tms[3];
         memcpy(c->timings, tms, sizeof tms);
     }
     return 0;
 }
+#undef MARFA_GET
 
 }  // namespace
 
@@ -913,7 +988,7 @@ int marfa_ctx_destroy(marfa_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_scratch(c->own);
-    marfa_ctx::Buf *bufs[] = {&c->lvl, &c->win, &c->out, &c->counts};
+    marfa_ctx::Buf *bufs[] = {&c->lvl, &c->win, &c->out, &c->counts, &c->adjout};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
@@ -1038,7 +1113,8 @@ int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *o
         if (ensure(c, c->out, total * 4)) return MARFA_E_HIP;
         out = (float *)c->out.p;
     }
-    rc = run(c, *job, L, w, out, st, true);
+    // phases are timed (and the host waits for them) unless the call is asynchronous
+    rc = run(c, *job, L, w, out, st, out_host != nullptr || !async);
     if (rc) return rc;
     if (out_host) {
         HIPCHK(hipMemcpyAsync(out_host, out, total * 4, hipMemcpyDeviceToHost, st));
@@ -1065,40 +1141,79 @@ int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, voi
     return 0;
 }
 
+// adjoint into device buffers: grad_lines [4][n], grad_levels [2][nlev], on stream st
+static int adjoint_dev(marfa_ctx *c, const marfa_job *job, const float *grad_out, hipStream_t st, double *grad_lines,
+                       double *grad_levels, bool timed) {
+    if (job->chi_id == MARFA_CHI_PERRIN && job->mol_code == 2)
+        return set_err(c, MARFA_E_ARG, "adjoint: the Perrin chi factor's temperature dependence is not differentiated");
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    LevelSet L = level_set(c, *job);
+    const int nlev = (int)L.P.size();
+    if (w.nrec == 0) {
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((8 * c->n + BLOCK - 1) / BLOCK, 65536)),
+                           dim3(BLOCK), 0, st, (int32_t *)grad_lines, 8 * c->n, 0);
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(BLOCK), 0, st, (int32_t *)grad_levels, (int64_t)4 * nlev, 0);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    AdjReq req;
+    req.gout = grad_out;
+    req.glines = grad_lines;
+    req.glevels = grad_levels;
+    // dL/dP_level and dL/dT_level: T also enters through pSelf = rho 10 k T / p0 (in the
+    // line pass) -- the density is held fixed
+    return run(c, *job, L, w, nullptr, st, timed, &req);
+}
+
 int marfa_pt_tables_adjoint(marfa_ctx *c, const marfa_job *job, const float *grad_out, void *stream,
                             double *grad_lines, double *grad_levels) {
     int rc = check_job(c, job);
     if (rc) return rc;
     if (!grad_out || !grad_lines || !grad_levels) return set_err(c, MARFA_E_ARG, "null argument");
     if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
-    if (job->chi_id == MARFA_CHI_PERRIN && job->mol_code == 2)
-        return set_err(c, MARFA_E_ARG, "adjoint: the Perrin chi factor's temperature dependence is not differentiated");
     (void)hipSetDevice(c->device);
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    Windows w;
-    compute_windows(c->h_nu, *job, w);
-    LevelSet L = level_set(c, *job);
-    const int nlev = (int)L.P.size();
-    std::fill(grad_lines, grad_lines + 4 * (size_t)c->n, 0.0);
-    std::fill(grad_levels, grad_levels + 2 * (size_t)nlev, 0.0);
-    if (w.nrec == 0) return 0;
-    std::vector<double> tp;
-    AdjReq req;
-    req.gout = grad_out;
-    req.tp = &tp;
-    rc = run(c, *job, L, w, nullptr, st, true, &req);
+    const size_t nlev = c->hP.size(), nline = (size_t)c->n * 4 * 8;
+    if (ensure(c, c->adjout, nline + nlev * 2 * 8 + 256)) return MARFA_E_HIP;
+    double *gl = (double *)c->adjout.p, *gv = (double *)((char *)c->adjout.p + ((nline + 255) & ~(size_t)255));
+    rc = adjoint_dev(c, job, grad_out, st, gl, gv, true);  // synchronous anyway: phases timed
     if (rc) return rc;
-    std::vector<double> acc((size_t)req.nl * 4);
-    HIPCHK(hipMemcpyAsync(acc.data(), c->s->gacc.p, acc.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(grad_lines, gl, nline, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(grad_levels, gv, nlev * 2 * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int64_t j = 0; j < req.nl; j++)
-        for (int q = 0; q < 4; q++) grad_lines[(size_t)q * c->n + req.lmin + j] = acc[(size_t)j * 4 + q];
-    // dL/dP_level and dL/dT_level; T also enters through pSelf = rho 10 k T / p0 (in the
-    // line pass) -- the density is held fixed
-    for (int l = 0; l < nlev; l++) {
-        grad_levels[l] = tp[l];
-        grad_levels[nlev + l] = tp[nlev + l];
-    }
+    return 0;
+}
+
+int marfa_pt_tables_adjoint_dev(marfa_ctx *c, const marfa_job *job, const float *grad_out, void *stream,
+                                double *grad_lines, double *grad_levels) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (!grad_out || !grad_lines || !grad_levels) return set_err(c, MARFA_E_ARG, "null argument");
+    if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
+    (void)hipSetDevice(c->device);
+    return adjoint_dev(c, job, grad_out, stream ? (hipStream_t)stream : c->stream, grad_lines, grad_levels, false);
+}
+
+int marfa_update_line_params(marfa_ctx *c, const float *S, const float *gair, const float *nT, const float *delta,
+                             void *stream) {
+    if (!c || !S || !gair || !nT || !delta) return set_err(c, MARFA_E_ARG, "null argument");
+    if (c->n == 0) return set_err(c, MARFA_E_STATE, "lines must be set");
+    (void)hipSetDevice(c->device);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    const size_t b = (size_t)c->n * 4;
+    HIPCHK(hipMemcpyAsync(c->d_S, S, b, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_gair, gair, b, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_nT, nT, b, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_delta, delta, b, hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+int marfa_ctx_info(const marfa_ctx *c, int64_t *nlines, int32_t *nlev, int32_t *device) {
+    if (!c) return MARFA_E_ARG;
+    if (nlines) *nlines = c->n;
+    if (nlev) *nlev = (int32_t)c->hP.size();
+    if (device) *device = c->device;
     return 0;
 }
 
@@ -1156,7 +1271,8 @@ int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
     // shifted centres for every level, then one workgroup per record
     const int64_t lmin = *std::min_element(w.lo.begin(), w.lo.end());
     const int64_t lmax = *std::max_element(w.hi.begin(), w.hi.end());
-    const int64_t nl = std::max<int64_t>(lmax - lmin, 1);
+    if (lmax <= lmin) return 0;  // every window empty: no samples (and no line is read)
+    const int64_t nl = lmax - lmin;
     hipStream_t st = c->stream;
     int64_t total = 0, total0 = 0;
     for (int l = 0; l < nlev; l++) {

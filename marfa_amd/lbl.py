@@ -65,6 +65,9 @@ def library() -> C.CDLL:
     L.marfa_pt_tables.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, C.c_int32]
     L.marfa_pt_tables_adjoint.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
     L.marfa_pt_tables_f64.argtypes = [vp, C.POINTER(_Job), vp, vp]
+    L.marfa_pt_tables_adjoint_dev.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
+    L.marfa_update_line_params.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.marfa_ctx_info.argtypes = [vp, i64p, i32p, i32p]
     L.marfa_batch_create.argtypes = [C.c_int, C.c_int32, C.c_double, C.POINTER(C.c_void_p)]
     L.marfa_batch_destroy.argtypes = [vp]
     L.marfa_batch_last_error.argtypes = [vp]
@@ -88,7 +91,8 @@ def library() -> C.CDLL:
 # every symbol the header declares (tests check the library exports them)
 EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_version",
            "marfa_set_tips", "marfa_set_lines", "marfa_set_levels", "marfa_windows",
-           "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint", "marfa_count_samples",
+           "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint",
+           "marfa_pt_tables_adjoint_dev", "marfa_update_line_params", "marfa_ctx_info", "marfa_count_samples",
            "marfa_grid_sums", "marfa_last_timings", "marfa_batch_create", "marfa_batch_destroy",
            "marfa_batch_last_error", "marfa_batch_set_tips", "marfa_batch_add_job", "marfa_batch_tune",
            "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records",

@@ -27,15 +27,9 @@ INFO_FILENAME = "info.txt"
 
 def get_pt_name(directory: Path, level: int) -> Path:
     """PT-table file of an atmospheric level (1-based): 1__.ptbin, 12_.ptbin, 123.ptbin
-    (handlers/constants.py:15-34; the labels app/main.f90:99-108 writes)."""
-    name = "___."
-    if level < 10:
-        name = f"{level}{name[1:]}"
-    elif level < 100:
-        name = f"{level}{name[2:]}"
-    else:
-        name = f"{level}{name[3:]}"
-    return Path(directory) / f"{name}{PT_EXTENSION}"
+    (handlers/constants.py:15-34; the labels app/main.f90:99-108 writes): the level number
+    padded with underscores to three characters."""
+    return Path(directory) / f"{str(level).ljust(3, '_')}.{PT_EXTENSION}"
 
 
 def record_number(a: float) -> int:
@@ -78,41 +72,50 @@ def write_pt_tables(directory, records, recnums, first_level: int = 1, levels=No
     return out
 
 
+def _record_span(v1: float, v2: float) -> tuple[int, int]:
+    """First and last record of the wavenumber range [v1, v2] as the handlers take it
+    (handlers/parsers.py:23-25): int(v1 / 10) .. int((v2 - 1) / 10)."""
+    return int(v1 / 10.0), int((v2 - 1) / 10.0)
+
+
+def _wavenumbers(first: int, last: int) -> np.ndarray:
+    """FP32 wavenumbers of the points of records first..last: 10 * record + i * 10/20480,
+    formed in FP64 and rounded once, as the handlers form them."""
+    step = RECORD_WV_SPAN / (POINTS_PER_RECORD - 1)
+    i = np.arange(POINTS_PER_RECORD, dtype=np.float64)
+    rec = np.arange(first, last + 1, dtype=np.int64)[:, None]
+    return (RECORD_WV_SPAN * rec + i[None, :] * step).astype(np.float32)
+
+
 def base_parser(pttable_file: Path, v1: float, v2: float) -> tuple[list[np.float32], list[np.float32]]:
     """Wavenumbers and values of records int(v1/10) .. int((v2-1)/10) of a PT-table file,
     as handlers/parsers.py:17-65 returns them (lists of np.float32): IndexError for a record
     past the end of the file; a missing file prints the reference's message and exits."""
-    start_record_number = int(v1 / 10.0)
-    end_record_number = int((v2 - 1) / 10.0)
-    num_records = end_record_number - start_record_number + 1
-    step = RECORD_WV_SPAN / (POINTS_PER_RECORD - 1)
-    vw_data: list[np.float32] = []
-    absorption_data: list[np.float32] = []
+    first, last = _record_span(v1, v2)
     try:
         size = os.path.getsize(pttable_file)
-        with open(pttable_file, "rb") as f:
-            for j in range(num_records):
-                record = start_record_number + j
-                seek_position = (record - 1) * RECORD_SIZE
-                if seek_position >= size:
-                    raise IndexError(f"Record number {record} exceeds a file size.")
-                f.seek(seek_position)
-                abs_data = np.frombuffer(f.read(RECORD_SIZE), dtype="<f4")
-                start = RECORD_WV_SPAN * record
-                vw_data.extend(np.float32(start + i * step) for i in range(POINTS_PER_RECORD))
-                absorption_data.extend(np.float32(x) for x in abs_data)
     except FileNotFoundError:
         print("Error: invalid number of atmospheric level")
         sys.exit()
-    return vw_data, absorption_data
+    # records are read in one piece once every one of them is known to start inside the file
+    past = [r for r in range(first, last + 1) if (r - 1) * RECORD_SIZE >= size]
+    if past:
+        raise IndexError(f"Record number {past[0]} exceeds a file size.")
+    n = last - first + 1
+    with open(pttable_file, "rb") as f:
+        f.seek((first - 1) * RECORD_SIZE)
+        raw = f.read(n * RECORD_SIZE)
+    if len(raw) < n * RECORD_SIZE:  # the handlers index every point of every record
+        raise IndexError(f"Record number {last} is cut short by the end of the file.")
+    vals = np.frombuffer(raw, dtype="<f4")
+    return list(_wavenumbers(first, last).reshape(-1)), list(vals)
 
 
 def read_pt_range(pttable_file: Path, v1: float, v2: float, device=None):
     """base_parser's points as tensors (wavenumber, value), FP32, on `device` if given:
     one read of the whole record range, the wavenumbers formed as base_parser forms them."""
     import torch
-    start = int(v1 / 10.0)
-    end = int((v2 - 1) / 10.0)
+    start, end = _record_span(v1, v2)
     n = end - start + 1
     size = os.path.getsize(pttable_file)
     if (end - 1) * RECORD_SIZE >= size:
@@ -120,10 +123,7 @@ def read_pt_range(pttable_file: Path, v1: float, v2: float, device=None):
     with open(pttable_file, "rb") as f:
         f.seek((start - 1) * RECORD_SIZE)
         vals = np.frombuffer(f.read(n * RECORD_SIZE), dtype="<f4").reshape(n, POINTS_PER_RECORD)
-    step = RECORD_WV_SPAN / (POINTS_PER_RECORD - 1)
-    i = np.arange(POINTS_PER_RECORD, dtype=np.float64)
-    wv = (RECORD_WV_SPAN * np.arange(start, end + 1, dtype=np.float64)[:, None] + i[None, :] * step).astype(np.float32)
-    wv_t = torch.from_numpy(wv.reshape(-1))
+    wv_t = torch.from_numpy(_wavenumbers(start, end).reshape(-1))
     val_t = torch.from_numpy(vals.reshape(-1).copy())
     if device is not None:
         wv_t, val_t = wv_t.to(device), val_t.to(device)

@@ -189,6 +189,30 @@ def config5(seed=SEED_BASE + 5, nlines=8_000_000, nlev=100) -> Job:
     return Job(lines, atm, 10.0, 20000.0, 250.0, mol_code=1, chi_id=0)
 
 
+def config2_window(nlines=400_000, nlev=6, width=100.0, seed=SEED_BASE + 2) -> Job:
+    """Configuration 2's laws at a reduced size: `nlines` banded lines over 500-10,000 cm-1,
+    `nlev` levels from 90 to 1e-4 atm (735 -> 170 K), cutoff 125, Pollack chi, and the
+    range cut to the `width` cm-1 (on the 10 cm-1 grid) that holds the most lines."""
+    job = config2(seed=seed, nlines=nlines, nlev=nlev)
+    h, _ = np.histogram(job.lines.nu, bins=np.arange(500.0, 10000.0 + 1e-9, 10.0))
+    k = int(np.argmax(np.convolve(h, np.ones(int(width // 10)), "valid")))
+    job.start_wv = 500.0 + 10.0 * k
+    job.end_wv = job.start_wv + width
+    return job
+
+
+def config5_window(density=40.0, nlev=5, start=10000.0, width=100.0, seed=SEED_BASE + 5) -> Job:
+    """Configuration 5's laws (H2O-161, S 10^U(-32,-18), E'' U(0, 8000), cutoff 250, no chi,
+    P 100 -> 1e-5 atm, T 750 -> 160 K, VMR 0.01) on the range [start, start + width) with
+    `density` lines per cm-1 over the range's reach (+-cutoff)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = start - 255.0, start + width + 255.0
+    nu = rng.uniform(lo, hi, int(density * (hi - lo)))
+    lines = make_lines(nu, rng, s_log=(-32.0, -18.0), e_max=8000.0, molIso=101)
+    atm = log_levels(nlev, 100.0, 1e-5, 750.0, 160.0, 0.01)
+    return Job(lines, atm, start, start + width, 250.0, mol_code=1, chi_id=0)
+
+
 def config3_jobs(seed=SEED_BASE + 3, njobs=256, max_lines=1_000_000) -> list:
     """256 independent jobs: N ~ 10^U(3,6) lines, 10-100 levels, windows of 20-500 cm-1 at
     random positions in 10-20,000 cm-1, molar mass from {18,44,16,28,48}, cutoff from

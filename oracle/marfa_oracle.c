@@ -53,7 +53,7 @@ static inline float f_of_bits(uint32_t b) {
 #define TREF_F 296.f             /* Constants.f90:24 */
 
 static const uint32_t WISO_BITS[124] = {
-#include "wiso_table.inc"
+#include "../marfa_amd/csrc/wiso_table.inc" /* bit patterns read from the reference, pinned by tests/golden/units.npz */
 };
 
 float mo_wiso(int iso) { return (iso >= 1 && iso <= 124) ? f_of_bits(WISO_BITS[iso - 1]) : 0.f; }

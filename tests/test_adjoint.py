@@ -100,7 +100,7 @@ def test_adjoint_operator_vs_fp64_autograd(reference_grads, prec):
     job, _, gk, gref = reference_grads
     model = PTTableModel(job, TIPS, precision=prec)
     try:
-        gl, gv = model.adjoint(gk.to("cuda", torch.float32))
+        gl, gv = (t.cpu().numpy() for t in model.adjoint(gk.to("cuda", torch.float32)))
     finally:
         model.close()
     compare([gl[0], gl[1], gl[2], gl[3], gv[0], gv[1]], gref, 1e-5 if prec == 1 else 1e-6, 1e-6)
@@ -135,7 +135,7 @@ def test_adjoint_deterministic():
         b = model.adjoint(g)
     finally:
         model.close()
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
 
 
 @pytest.mark.gpu
@@ -145,7 +145,7 @@ def test_adjoint_rejects_perrin():
     model = PTTableModel(adjoint_job(n=500, nlev=1, chi=3), TIPS)
     try:
         rec = model.records()
-        with pytest.raises(lbl.MarfaError):
+        with pytest.raises(RuntimeError, match="Perrin"):
             model.adjoint(torch.ones_like(rec))
     finally:
         model.close()

@@ -209,14 +209,20 @@ def test_q3_line_at_right_edge_adds_nothing():
 
 
 def test_cutoff_asymmetry():
-    # left wing kept at d == cutoff (strict '>' test), right wing dropped at d == cutoff
-    for nu, want in ((3975.0, 21), (4035.0, 0)):
-        j = single_line_job(nu)
-        oj = po.OracleJob(j, TIPS)
-        # the first window starts after startWV - cutoff: pin the record directly
-        rk = np.zeros(20481)
-        _, cnt = oj.record(po.F64, 0, 0)
-        assert cnt.samples == 0 or cnt.samples == want
+    """leftLBL keeps a line at d == cutoff (its test is d > cutOff, src/LineGridCalc.f90:23),
+    rightLBL drops it (d >= cutOff, :508-509).  The windows of app/main.f90 never hand the
+    left one over (the starting line is the first with nu > startWV - cutOff), so the deposit
+    is called directly on a window holding just that line."""
+    for nu, want in ((3975.0, 21), (4035.0, 0), (3975.5, 21), (4034.5, 21)):
+        oj = po.OracleJob(single_line_job(nu), TIPS)
+        for prec in (po.R32, po.F64):
+            rk = np.zeros(20481)
+            cnt = po.Counts()
+            po.lib().mo_record(prec, po.C.byref(oj.lines), po.C.byref(oj.job), po._ptr(oj.tips), float(oj.P[0]),
+                               float(oj.T[0]), float(oj.rho[0]), 4000.0, 0, 1,
+                               rk.ctypes.data_as(po.C.POINTER(po.C.c_double)), None, po.C.byref(cnt))
+            assert cnt.samples == want, (nu, prec, cnt.samples)
+            assert (np.count_nonzero(rk) > 0) == (want > 0)
 
 
 @pytest.mark.parametrize("case", sorted(gen_golden.record_cases()))
