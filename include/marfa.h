@@ -30,6 +30,7 @@
 #ifndef MARFA_H
 #define MARFA_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -119,6 +120,18 @@ int marfa_lbl_scheme(marfa_ctx *ctx, int64_t *line_idx, double cap_wv, double st
  * out_host is NULL and async != 0; an asynchronous call records no phase timings. */
 int marfa_pt_tables(marfa_ctx *ctx, const marfa_job *job, float *out_dev, float *out_host,
                     void *stream, int32_t async);
+
+/* marfa_pt_tables without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The caller passes out_dev (device, the layout of
+ * marfa_pt_tables); the inputs are the context's (set beforehand).  The kernels' scratch is
+ * a fixed array of MARFA_NOALLOC_SCRATCH_BYTES in the library, one per GPU, held until the
+ * process ends; levels run in batches that fit it, and a job whose single level needs more
+ * is rejected with MARFA_E_ARG (message: marfa_last_error).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads) race; calls on
+ * one stream are ordered.  The call enqueues its work on stream and returns (no timings). */
+#define MARFA_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int marfa_pt_tables_noalloc(marfa_ctx *ctx, const marfa_job *job, float *out_dev, void *stream);
 
 /* marfa_pt_tables with FP64 records (no counterpart in the reference, whose records are
  * FP32): out_dev (device) receives the records before their rounding to FP32, for

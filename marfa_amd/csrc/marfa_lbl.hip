@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -495,6 +496,35 @@ __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long lon
 
 }  // namespace
 
+// ====================================================================== fixed scratch
+// Scratch of marfa_pt_tables_noalloc, for callers that cannot allocate device memory: one
+// array per GPU the library runs on, held until the process ends.  In host code the
+// array's name is not a device address; its address on each device is looked up once, with
+// that device current (fixed_scratch_of), and only that pointer is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = MARFA_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device (which must be `device`)
+hipError_t fixed_scratch_of(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
+
 // ====================================================================== host side
 struct marfa_ctx {
     int device = 0;
@@ -529,6 +559,8 @@ struct marfa_ctx {
     hipEvent_t ev[5] = {};
     float timings[5] = {0, 0, 0, 0, 0};
     double *dbg_grids = nullptr;
+    void *fixed_scratch = nullptr;  // this device's `scratch` (marfa_pt_tables_noalloc)
+    std::string fixed_err;
 };
 
 namespace {
@@ -976,6 +1008,9 @@ int marfa_ctx_create(int device, marfa_ctx **out) {
         return MARFA_E_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
+    // the allocation-free entry point's scratch on this device (looked up once per device)
+    const hipError_t e = fixed_scratch_of(device, &c->fixed_scratch);
+    if (e != hipSuccess) c->fixed_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
     *out = c;
     return 0;
 }
@@ -1123,6 +1158,24 @@ int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *o
         HIPCHK(hipStreamSynchronize(st));
     }
     return 0;
+}
+
+int marfa_pt_tables_noalloc(marfa_ctx *c, const marfa_job *job, float *out_dev, void *stream) {
+    int rc = check_job(c, job);
+    if (rc) return rc;
+    if (!out_dev) return set_err(c, MARFA_E_ARG, "out_dev is required: this entry point allocates nothing");
+    if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
+    if (!c->fixed_scratch) return set_err(c, MARFA_E_HIP, "no fixed scratch on this device: " + c->fixed_err);
+    (void)hipSetDevice(c->device);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    Windows w;
+    compute_windows(c->h_nu, *job, w);
+    LevelSet L = level_set(c, *job);
+    Arena ar;
+    ar.base = (char *)c->fixed_scratch;
+    ar.size = NOALLOC_SCRATCH_BYTES;
+    // no phase timing (no events to wait on): the call only enqueues its copies and kernels
+    return run(c, *job, L, w, out_dev, st, false, nullptr, nullptr, false, &ar);
 }
 
 int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, void *stream) {

@@ -65,6 +65,7 @@ def library() -> C.CDLL:
     L.marfa_pt_tables.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, C.c_int32]
     L.marfa_pt_tables_adjoint.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
     L.marfa_pt_tables_f64.argtypes = [vp, C.POINTER(_Job), vp, vp]
+    L.marfa_pt_tables_noalloc.argtypes = [vp, C.POINTER(_Job), vp, vp]
     L.marfa_pt_tables_adjoint_dev.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
     L.marfa_update_line_params.argtypes = [vp, vp, vp, vp, vp, vp]
     L.marfa_ctx_info.argtypes = [vp, i64p, i32p, i32p]
@@ -93,6 +94,7 @@ EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_v
            "marfa_set_tips", "marfa_set_lines", "marfa_set_levels", "marfa_windows",
            "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint",
            "marfa_pt_tables_adjoint_dev", "marfa_update_line_params", "marfa_ctx_info", "marfa_count_samples",
+           "marfa_pt_tables_noalloc",
            "marfa_grid_sums", "marfa_last_timings", "marfa_batch_create", "marfa_batch_destroy",
            "marfa_batch_last_error", "marfa_batch_set_tips", "marfa_batch_add_job", "marfa_batch_tune",
            "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records",
@@ -243,6 +245,13 @@ class LBLContext:
         j = spec._c()
         self._check(self.lib.marfa_pt_tables(self.h, C.byref(j), C.c_void_p(out_ptr), None,
                                              C.c_void_p(stream) if stream else None, 0 if sync else 1))
+
+    def pt_tables_noalloc(self, spec: JobSpec, out_ptr: int, stream: int = 0):
+        """marfa_pt_tables_noalloc: the records into an existing device buffer, with the
+        library's fixed scratch (no device allocation); enqueued on stream, not waited for."""
+        j = spec._c()
+        self._check(self.lib.marfa_pt_tables_noalloc(self.h, C.byref(j), C.c_void_p(out_ptr),
+                                                     C.c_void_p(stream) if stream else None))
 
     def pt_tables_torch(self, spec: JobSpec):
         import torch
