@@ -145,7 +145,7 @@ int marfa_pt_tables_f64(marfa_ctx *ctx, const marfa_job *job, double *out_dev, v
  * pSelf = rho 10 k T / p0 follows T).  The deposit geometry (which nodes each line samples)
  * is held fixed: the derivative is that of the piecewise-smooth model between its
  * decisions.  The level's grid sums are recomputed (not stored by the forward call).
- * Not available for the Perrin chi factor (MARFA_E_ARG). */
+ * The Perrin chi factor's temperature dependence enters dL/dT. */
 int marfa_pt_tables_adjoint(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
                             double *grad_lines, double *grad_levels);
 /* marfa_pt_tables_adjoint with the gradients in device memory (grad_lines [4][n], grad_levels

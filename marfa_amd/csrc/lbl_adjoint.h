@@ -34,7 +34,7 @@ struct LineAdj {
     const LineShape<R> *p;
     const ShapeCtl *ctl;
     double u;         // nu* - origin
-    double aS, aX, aD, aL;
+    double aS, aX, aD, aL, aT;
     __device__ inline void acc(double x, float gs) {
         if (gs == 0.f) return;
         const SampleGrad<R> s = shape_grad<R>(u - x, *p, *ctl);
@@ -42,6 +42,7 @@ struct LineAdj {
         aX += (double)gs * (double)s.dX;
         aD += (double)gs * (double)s.dgD;
         aL += (double)gs * (double)s.dgL;
+        aT += (double)gs * (double)s.dT;
     }
     // positions q = qlo..qhi (steps of H_k/2) of a run of cells of grid k: even q is the L
     // edge of cell q/2 (q > qlo) and the P edge of cell q/2+1 (q < qhi); odd q a mid point
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
         LineAdj<R> w;
         w.p = &p;
         w.ctl = &ctl;
-        w.aS = w.aX = w.aD = w.aL = 0.0;
+        w.aS = w.aX = w.aD = w.aL = w.aT = 0.0;
         // records whose window [lo, hi) holds line i: hi is non-decreasing in r
         int r = 0, hiR = a.nrec;
         while (r < hiR) {
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
         const double dQ = ((double)row[ti] - (double)row[ti - 1]) / 2.0;
         const double c2 = (double)MARFA_C2_F, x = c2 * b.nu[i] / T;
         const double dlnS = -dQ / Q + c2 * E / (T * T) - (x / T) * exp(-x) / (1.0 - exp(-x));
-        cT = w.aS * dlnS + w.aD * gD / (2.0 * T) + w.aL * (-n * gL / T + fT * (gs - ga) * ps / T);
+        cT = w.aS * dlnS + w.aD * gD / (2.0 * T) + w.aL * (-n * gL / T + fT * (gs - ga) * ps / T) + w.aT;
         cP = dnu * dl + w.aL * fT * ga;
     }
     // per-block partial sums of the level gradients, fixed order

@@ -154,6 +154,7 @@ struct ShapeCtl {
     int shape_id, chi_id, mol;
     float B1, B2;     // Perrin temperature coefficients of the level (FP32 as the reference)
     double B1d, B2d;  // same in FP64
+    double dB1, dB2;  // their temperature derivatives (the adjoint's Perrin term)
 };
 
 template <typename R>
@@ -368,6 +369,7 @@ __device__ inline R dchi_factor(int chi, R ax, R B1, R B2, R chiv) {
 template <typename R>
 struct SampleGrad {
     R v, dX, dgD, dgL;
+    R dT;  // at fixed X, gD, gL: through the Perrin chi factor's temperature coefficients
 };
 
 template <typename R>
@@ -389,6 +391,15 @@ __device__ inline void chilor_grad(R X, const LineShape<R> &p, const ShapeCtl &c
     o.dX = R(-2) * X * lor * inv * chi + lor * dchi * sg;
     o.dgD = R(0);
     o.dgL = p.S / R(MARFA_PI_F) * (X * X - p.gl2) * inv * inv * chi;
+    // Perrin (src/ChiFactors.f90:82-122): chi = exp(e(|X|, B1(T), B2(T))), d chi/dT = chi de/dT
+    // with e = -B1 (|X| - 3), -27 B1 - B2 (|X| - 30) or -27 B1 - 90 B2 - B3 (|X| - 120)
+    o.dT = R(0);
+    if (c.mol == 2 && c.chi_id == 3 && ax > R(3)) {
+        const R dB1 = (R)c.dB1, dB2 = (R)c.dB2;
+        const R de = ax < R(30) ? -dB1 * (ax - R(3)) : ax < R(120) ? -R(27) * dB1 - dB2 * (ax - R(30))
+                                                                 : -R(27) * dB1 - R(90) * dB2;
+        o.dT = o.v * de;
+    }
 }
 
 // Same region decisions as shape_eval / voigt_regions, value and derivatives of the branch
@@ -414,6 +425,7 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
     else if (VX + VY >= 5.5) region = 2;
     else if (VX <= 1.0 || VY >= (double)0.02f) region = 3;
     else region = 4;
+    o.dT = R(0);  // chi enters only the chi-corrected Lorentz and asymptote 1
     if (region < 4) {
         const Dual2<R> K = kuntz_rational<Dual2<R>>(region, Dual2<R>(vy, R(0), R(1)), Dual2<R>(vx * vx, R(1), R(0)));
         o.v = K.v * p.normc;
@@ -428,6 +440,7 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
         o.dX = o.dX * f + o.v * df * p.xs * sg;
         o.dgD = o.v * df * (-vx * p.igd);
         o.dgL = o.dgL * f;
+        o.dT = o.dT * f;
         o.v = o.v * f;
         return o;
     }

@@ -104,6 +104,8 @@ __device__ inline ShapeCtl make_ctl(const KArgs &a, int lev) {
     c.B2 = 0.0526f * expf(-(0.00152f * T));
     c.B1d = (double)0.0888f - (double)0.16f * exp(-((double)0.00410f * (double)T));
     c.B2d = (double)0.0526f * exp(-((double)0.00152f * (double)T));
+    c.dB1 = (double)0.16f * (double)0.00410f * exp(-((double)0.00410f * (double)T));
+    c.dB2 = -(double)0.0526f * (double)0.00152f * exp(-((double)0.00152f * (double)T));
     return c;
 }
 
@@ -1197,8 +1199,6 @@ int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, voi
 // adjoint into device buffers: grad_lines [4][n], grad_levels [2][nlev], on stream st
 static int adjoint_dev(marfa_ctx *c, const marfa_job *job, const float *grad_out, hipStream_t st, double *grad_lines,
                        double *grad_levels, bool timed) {
-    if (job->chi_id == MARFA_CHI_PERRIN && job->mol_code == 2)
-        return set_err(c, MARFA_E_ARG, "adjoint: the Perrin chi factor's temperature dependence is not differentiated");
     Windows w;
     compute_windows(c->h_nu, *job, w);
     LevelSet L = level_set(c, *job);

@@ -70,6 +70,52 @@ static const double HK[10] = {1.0, 0.5, 0.25, 0.125, 0.0625, 0.03125, 0.015625,
 #define HFINE 0.00048828125 /* H = H9/4 */
 static inline int gbase(int k) { return 30 * ((1 << k) - 1); } /* P block of level k */
 
+/* ------------------------------------------------------------------ decision trace */
+/* For the adjoint's finite-difference checks: while the trace is on, the F64 deposits hash
+ * every grid node they add a sample to (in order: the selection), the F64 profile every
+ * Voigt region and chi band it takes, and the FP64 cascade every limiter choice.  Two
+ * evaluations with equal hashes took the same branches, so a difference quotient between
+ * them does not straddle a discontinuity of the piecewise-smooth model.  Single-threaded. */
+static struct {
+    uint64_t sel, reg, lim;
+    int on;
+} g_tr;
+static inline uint64_t tr_mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+    return h * 0xBF58476D1CE4E5B9ULL;
+}
+static inline int tr_node(int i) {
+    if (g_tr.on) g_tr.sel = tr_mix(g_tr.sel, (uint64_t)i);
+    return i;
+}
+static inline void tr_reg(int code) {
+    if (g_tr.on) g_tr.reg = tr_mix(g_tr.reg, (uint64_t)code);
+}
+static inline int tr_lim(int lim) {
+    if (g_tr.on) g_tr.lim = tr_mix(g_tr.lim, (uint64_t)lim);
+    return lim;
+}
+void mo_trace_begin(void) {
+    g_tr.sel = 1;
+    g_tr.reg = 2;
+    g_tr.lim = 3;
+    g_tr.on = 1;
+}
+void mo_trace_end(uint64_t out[3]) {
+    out[0] = g_tr.sel;
+    out[1] = g_tr.reg;
+    out[2] = g_tr.lim;
+    g_tr.on = 0;
+}
+/* band of chi_id at |X| (src/ChiFactors.f90:29-122): 0 where chi = 1 */
+static int chi_band_of(int chi_id, double ax, int mol) {
+    if (mol != 2) return 0;
+    if (chi_id == 1) return ax > 3. ? (ax <= 150. ? 1 : ax <= 300. ? 2 : 3) : 0;
+    if (chi_id == 2) return ax >= 3. ? (ax < 10. ? 1 : 2) : 0;
+    if (chi_id == 3) return ax > 3. ? (ax < 30. ? 1 : ax < 120. ? 2 : 3) : 0;
+    return 0;
+}
+
 /* ------------------------------------------------------------------ chi factors */
 /* src/ChiFactors.f90:18-122.  R32 replays the reference's mixed precision:
  * abs(X) and the exponentials are FP64, the result is FP32. */
@@ -251,6 +297,7 @@ typedef struct {
 } f64_line;
 
 static inline double f64_chilor(double X, const f64_line *p) {
+    if (g_tr.on) tr_reg(100 + chi_band_of(p->chi_id, fabs(X), p->mol));
     double lor = p->gL / ((double)PI_F * (X * X + p->gL * p->gL)) * p->S;
     return lor * mo_chi_f64(p->chi_id, X, p->mol, p->T);
 }
@@ -270,6 +317,7 @@ static double f64_voigt(double X, const f64_line *p) {
     double V2 = VX * VX, num, den;
 #define D(x) ((double)(x))
     if (VX + VY >= 15.) {
+        tr_reg(1);
         double Y2 = VY * VY;
         double A1 = (D(0.2820948f) + D(0.5641896f) * Y2) * VY;
         double B1 = D(0.5641896f) * VY;
@@ -278,6 +326,7 @@ static double f64_voigt(double X, const f64_line *p) {
         num = A1 + B1 * V2;
         den = A2 + B2 * V2 + V2 * V2;
     } else if (VX + VY >= 5.5) {
+        tr_reg(2);
         double Y2 = VY * VY;
         double A3 = VY * (((D(0.56419f) * Y2 + D(3.10304f)) * Y2 + D(4.65456f)) * Y2 + D(1.05786f));
         double B3 = VY * ((D(1.69257f) * Y2 + D(0.56419f)) * Y2 + D(2.962f));
@@ -290,6 +339,7 @@ static double f64_voigt(double X, const f64_line *p) {
         num = ((D3 * V2 + C3) * V2 + B3) * V2 + A3;
         den = (((V2 + D4) * V2 + C4) * V2 + B4) * V2 + A4;
     } else if (VX <= 1.0 || VY >= D(0.02f)) {
+        tr_reg(3);
         double Y = VY;
         double A5 = ((((((((D(0.564224f) * Y + D(7.55895f)) * Y + D(49.5213f)) * Y + D(204.510f)) * Y +
                         D(581.746f)) * Y + D(1174.8f)) * Y + D(1678.33f)) * Y + D(1629.76f)) * Y +
@@ -312,9 +362,11 @@ static double f64_voigt(double X, const f64_line *p) {
         num = (((E5 * V2 + D5) * V2 + C5) * V2 + B5) * V2 + A5;
         den = ((((V2 + E6) * V2 + D6) * V2 + C6) * V2 + B6) * V2 + A6;
     } else {
+        tr_reg(4);
         if (VX > 5.) return f64_chilor(X, p) * (1. + 1.5 / (VX * VX));
         if (VX > D(SQRT14_F)) {
             int I = (int)(VX / 0.5 - D(1.00001f));
+            tr_reg(10 + I);
             double F = 2. * (D(ASYM_W[I - 1]) * (D(ASYM_U[I]) - VX) + D(ASYM_W[I]) * (VX - D(ASYM_U[I - 1])));
             return f64_doppler(X, gD, S) + (gL / ((double)PI_F * (X * X)) * (1. + F)) * S;
         }
@@ -390,7 +442,7 @@ void mo_spec_f64(const float *tips, double nu, float S, float gair, float gself,
 
 /* ------------------------------------------------------------------ cascade */
 /* src/Grids.f90:80-139.  grids: level-major P|M|L blocks, rk 1..20481 stored 0-based. */
-#define CASCADE_BODY(REAL, C375, C75, C125, C5)                                              \
+#define CASCADE_BODY(REAL, C375, C75, C125, C5, LIM)                                         \
     for (int k = 0; k < 9; k++) {                                                            \
         const int n = NTK[k];                                                                \
         const REAL *cP = g + gbase(k), *cM = cP + n, *cL = cM + n;                            \
@@ -399,12 +451,12 @@ void mo_spec_f64(const float *tips, double nu, float S, float gair, float gself,
             int i = 2 * j;                                                                   \
             fP[i] = fP[i] + cP[j];                                                           \
             REAL A = C375 * cP[j] + C75 * cM[j] - C125 * cL[j];                              \
-            if (A < cP[j] || A < cM[j]) A = C5 * (cP[j] + cM[j]);                            \
+            if (LIM(A < cP[j] || A < cM[j])) A = C5 * (cP[j] + cM[j]);                       \
             fM[i] = fM[i] + A;                                                               \
             fL[i] = fL[i] + cM[j];                                                           \
             fP[i + 1] = fP[i + 1] + cM[j];                                                   \
             A = C375 * cL[j] + C75 * cM[j] - C125 * cP[j];                                   \
-            if (A < cL[j] || A < cM[j]) A = C5 * (cL[j] + cM[j]);                            \
+            if (LIM(A < cL[j] || A < cM[j])) A = C5 * (cL[j] + cM[j]);                       \
             fM[i + 1] = fM[i + 1] + A;                                                       \
             fL[i + 1] = fL[i + 1] + cL[j];                                                   \
         }                                                                                    \
@@ -413,18 +465,19 @@ void mo_spec_f64(const float *tips, double nu, float S, float gair, float gself,
         const REAL *P9 = g + gbase(9), *M9 = P9 + 5120, *L9 = M9 + 5120;                     \
         for (int j = 0; j < 5120; j++) {                                                     \
             REAL A = P9[j] * C375 + M9[j] * C75 - L9[j] * C125;                              \
-            if (A < P9[j] || A < M9[j]) A = C5 * (P9[j] + M9[j]);                            \
+            if (LIM(A < P9[j] || A < M9[j])) A = C5 * (P9[j] + M9[j]);                       \
             rk[4 * j + 1] = rk[4 * j + 1] + A;                                               \
             rk[4 * j + 2] = rk[4 * j + 2] + M9[j];                                           \
             A = L9[j] * C375 + M9[j] * C75 - P9[j] * C125;                                   \
-            if (A < L9[j] || A < M9[j]) A = C5 * (L9[j] + M9[j]);                            \
+            if (LIM(A < L9[j] || A < M9[j])) A = C5 * (L9[j] + M9[j]);                       \
             rk[4 * j + 3] = rk[4 * j + 3] + A;                                               \
             rk[4 * j + 4] = rk[4 * j + 4] + L9[j];                                           \
         }                                                                                    \
     }
 
-static void cascade_r32(float *g, float *rk) { CASCADE_BODY(float, 0.375f, 0.75f, 0.125f, 0.5f) }
-static void cascade_f64(double *g, double *rk) { CASCADE_BODY(double, 0.375, 0.75, 0.125, 0.5) }
+#define NO_TRACE(x) (x)
+static void cascade_r32(float *g, float *rk) { CASCADE_BODY(float, 0.375f, 0.75f, 0.125f, 0.5f, NO_TRACE) }
+static void cascade_f64(double *g, double *rk) { CASCADE_BODY(double, 0.375, 0.75, 0.125, 0.5, tr_lim) }
 
 void mo_cascade_r32(const float *grids, float *rk) {
     float *g = (float *)malloc(sizeof(float) * MO_NGRID);
@@ -451,20 +504,24 @@ void mo_cascade_f64(const double *grids, double *rk) {
 #define GLINE r32_line
 #define GSHAPE r32_shape
 #define GNAME(x) x##_r32
+#define GTRACE(i) (i)
 #include "oracle_deposit.inc"
 #undef GREAL
 #undef GLINE
 #undef GSHAPE
 #undef GNAME
+#undef GTRACE
 #define GREAL double
 #define GLINE f64_line
 #define GSHAPE f64_shape
 #define GNAME(x) x##_f64
+#define GTRACE(i) tr_node(i)
 #include "oracle_deposit.inc"
 #undef GREAL
 #undef GLINE
 #undef GSHAPE
 #undef GNAME
+#undef GTRACE
 
 /* ------------------------------------------------------------------ windows */
 int mo_windows(const mo_lines *Ls, const mo_job *J, int max_rec, int64_t *lo, int64_t *hi,

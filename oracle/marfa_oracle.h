@@ -77,6 +77,12 @@ int mo_pt_tables(int precision, const mo_lines *L, const mo_job *J, const float 
 int mo_direct(const mo_lines *L, const mo_job *J, const float *tips, float P, float T,
               float rho, double a, int64_t lo, int64_t hi, int i0, int stride, double *out);
 
+/* Decision trace of the F64 path (for finite-difference checks): between begin and end,
+ * mo_record / mo_cascade_f64 in MO_F64 hash the grid nodes that receive samples (out[0]),
+ * the Voigt regions and chi bands taken (out[1]) and the cascade limiter choices (out[2]). */
+void mo_trace_begin(void);
+void mo_trace_end(uint64_t out[3]);
+
 /* Unit-level entry points for golden-vector tests. */
 float mo_voigt_r32(double X, float gD, float gL, float S, int shape_id, int chi_id,
                    int mol, float T);

@@ -85,8 +85,26 @@ def lib():
         L.mo_direct.argtypes = [C.POINTER(_Lines), C.POINTER(_Job), C.c_void_p, C.c_float,
                                 C.c_float, C.c_float, C.c_double, C.c_int64, C.c_int64,
                                 C.c_int, C.c_int, dp]
+        L.mo_trace_begin.argtypes = []
+        L.mo_trace_end.argtypes = [C.c_void_p]
         _lib = L
     return _lib
+
+
+class Trace:
+    """Decision trace of the F64 path (mo_trace_begin / mo_trace_end): `with Trace() as t:`
+    around mo_record / mo_cascade_f64 calls; afterwards t.hashes = (selection, regions,
+    limiter).  Equal hashes: the same branches were taken."""
+
+    def __enter__(self):
+        lib().mo_trace_begin()
+        return self
+
+    def __exit__(self, *exc):
+        out = np.zeros(3, np.uint64)
+        lib().mo_trace_end(_ptr(out))
+        self.hashes = tuple(int(x) for x in out)
+        return False
 
 
 def _ptr(a):
@@ -161,6 +179,51 @@ class OracleJob:
                         rk.ctypes.data_as(C.POINTER(C.c_double)), _ptr(g), None)
         return g
 
+    def with_param(self, field, i, value) -> "OracleJob":
+        """A copy with one line parameter (S, gair, nT, delta: field name of synth.Lines)
+        replaced: line i's value."""
+        import copy
+        o = copy.copy(self)
+        names = ("nu", "S", "gair", "gself", "elow", "nT", "delta", "molIso")
+        k = names.index(field)
+        o._keep = list(self._keep)
+        o._keep[k] = self._keep[k].copy()
+        o._keep[k][i] = value
+        o.lines = _Lines(len(self._keep[0]), *[a.ctypes.data for a in o._keep])
+        return o
+
+    def with_level(self, lev, T=None, P=None) -> "OracleJob":
+        """A copy with level lev's temperature and/or pressure replaced (density kept)."""
+        import copy
+        o = copy.copy(self)
+        o.T, o.P = self.T.copy(), self.P.copy()
+        if T is not None:
+            o.T[lev] = T
+        if P is not None:
+            o.P[lev] = P
+        return o
+
+    def decisions_line(self, lev, i):
+        """Selection and region hashes of line i's own deposits in every record whose window
+        holds it, at level lev (the line alone: no cascade)."""
+        recs = [r for r in range(self.nrec) if self.lo[r] <= i < self.hi[r]]
+        with Trace() as t:
+            for r in recs:
+                g = np.zeros(NGRID + NT, np.float64)
+                rk = np.zeros(NT, np.float64)
+                lib().mo_record(F64, C.byref(self.lines), C.byref(self.job), _ptr(self.tips),
+                                float(self.P[lev]), float(self.T[lev]), float(self.rho[lev]),
+                                float(self.origin[r]), int(i), int(i) + 1,
+                                rk.ctypes.data_as(C.POINTER(C.c_double)), _ptr(g), None)
+        return t.hashes[:2], recs
+
+    def decisions_level(self, lev):
+        """Selection, region and limiter hashes of every record of level lev (all lines)."""
+        with Trace() as t:
+            for r in range(self.nrec):
+                self.record(F64, lev, r)
+        return t.hashes
+
     def direct(self, lev, r, i0=0, stride=1):
         npts = len(range(i0, NT, stride))
         out = np.zeros(npts, np.float64)
@@ -194,6 +257,15 @@ def spec(precision, tips, nu, S, gair, gself, E, n, molIso, delta, P, T, rho):
     f(_ptr(tips), nu, S, gair, gself, E, n, molIso, delta, P, T, rho,
       out.ctypes.data_as(C.POINTER(C.c_double)))
     return out
+
+
+def limiter_hash(grids_list):
+    """Limiter-choice hash of the FP64 cascade of each pre-cascade grid set (NGRID + NT
+    doubles each, e.g. the GPU's marfa_grid_sums output), in order."""
+    with Trace() as t:
+        for g in grids_list:
+            cascade(F64, np.asarray(g[:NGRID], np.float64), np.asarray(g[NGRID:], np.float64))
+    return t.hashes[2]
 
 
 def cascade(precision, grids, rk):

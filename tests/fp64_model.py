@@ -90,6 +90,14 @@ def _chi(ax, chi_id, mol, T):
     if chi_id == 2:
         return torch.where(ax >= 3, torch.where(ax < 10, f(1.35) * torch.exp(-ax / 10.0),
                                                 f(0.614) * torch.exp(-ax / 47.0)), one)
+    if chi_id == 3:  # Perrin: temperature-dependent exponents (src/ChiFactors.f90:82-122)
+        T = torch.as_tensor(T, dtype=ax.dtype)
+        B1 = f(0.0888) - f(0.16) * torch.exp(-f(0.0041) * T)
+        B2 = f(0.0526) * torch.exp(-f(0.00152) * T)
+        B3 = f(0.0232)
+        return torch.where(ax > 3, torch.where(ax < 30, torch.exp(-B1 * (ax - 3.0)),
+                                               torch.where(ax < 120, torch.exp(-B1 * 27.0 - B2 * (ax - 30.0)),
+                                                           torch.exp(-B1 * 27.0 - B2 * 90.0 - B3 * (ax - 120.0)))), one)
     raise NotImplementedError("chi factor")
 
 
@@ -237,7 +245,7 @@ class Model:
         li = d["line"]
         org = torch.as_tensor(self.geo.origin)[d["rec"]]
         X = (nus[li] - org) - d["pos"]
-        return shape(X, gD[li], gL[li], ST[li], self.job.shape_id, self.job.chi_id, self.job.mol_code, float(T[lev].detach()))
+        return shape(X, gD[li], gL[li], ST[li], self.job.shape_id, self.job.chi_id, self.job.mol_code, T[lev])
 
     def records(self, params):
         out = []
