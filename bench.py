@@ -35,68 +35,165 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def accuracy(job, tips, gpu_records, picks):
-    """max / rms relative error of GPU records against the FP64 oracle (gridded)."""
+def _stats(errs):
+    e = np.concatenate(errs) if errs else np.zeros(1)
+    return {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e ** 2)))}
+
+
+def accuracy(job, tips, gpu_records, picks, exact_stride=64):
+    """Relative errors of GPU records on sampled (level, record) pairs:
+      * against the FP64 gridded oracle (MO_F64, the reference's algorithm in FP64), every point;
+      * against the un-gridded FP64 sum (mo_direct, O_exact) at every exact_stride-th point,
+        beside the deviation of the FP64 gridded oracle and of the reference-precision
+        restatement (MO_R32, bit-exact with the reference) from it: the gridding's own error."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     oj = po.OracleJob(job, tips)
-    errs, maxe = [], 0.0
+    errs, ex_gpu, ex_f64, ex_r32 = [], [], [], []
+    tiny = np.finfo(np.float32).tiny
     for lev, r in picks:
         ref, _ = oj.record(po.F64, lev, r)
         g = gpu_records[(lev, r)].astype(np.float64)
-        m = np.abs(ref) >= np.finfo(np.float32).tiny
-        e = np.abs(g[m] - ref[m]) / np.abs(ref[m])
-        if e.size:
-            errs.append(e)
-            maxe = max(maxe, float(e.max()))
+        m = np.abs(ref) >= tiny
+        if m.any():
+            errs.append(np.abs(g[m] - ref[m]) / np.abs(ref[m]))
         # records without absorption (spectral gaps wider than twice the cutoff) must be 0
-        assert np.all(g[~m] == 0) or np.all(np.abs(g[~m]) < np.finfo(np.float32).tiny * 4)
-    e = np.concatenate(errs) if errs else np.zeros(1)
-    return {"max_rel_err": maxe, "rms_rel_err": float(np.sqrt(np.mean(e ** 2))),
-            "records_checked": len(picks), "vs": "FP64 gridded oracle (oracle/marfa_oracle.c, MO_F64)"}
+        assert np.all(g[~m] == 0) or np.all(np.abs(g[~m]) < tiny * 4)
+        ex = oj.direct(lev, r, 0, exact_stride)
+        r32, _ = oj.record(po.R32, lev, r)
+        me = np.abs(ex) >= tiny
+        for dst, x in ((ex_gpu, g), (ex_f64, ref), (ex_r32, r32)):
+            xs = x[::exact_stride]
+            dst.append(np.abs(xs[me] - ex[me]) / np.abs(ex[me]))
+    st = _stats(errs)
+    return {"max_rel_err": st["max"], "rms_rel_err": st["rms"], "records_checked": len(picks),
+            "vs": "FP64 gridded oracle (oracle/marfa_oracle.c, MO_F64)",
+            "vs_exact": {"every": exact_stride, "gpu": _stats(ex_gpu), "fp64_gridded_oracle": _stats(ex_f64),
+                         "reference_precision_gridded": _stats(ex_r32),
+                         "note": "deviation from the un-gridded FP64 Voigt sum (mo_direct): the gridded algorithm's "
+                                 "own error, shared by the reference, the FP64 oracle and both GPU forms"}}
 
 
-def cpu_baseline(job, ctx_counts_full, target_samples=1.0e9, level=0):
-    """Time the reference's LBL modules (oracle/_ref/ref_driver, one thread) on the first
-    subintervals of one level, as many as make about `target_samples` profile samples
-    (~15-20 s of CPU), and scale to the full workload by profile samples."""
+def config_errors(tips):
+    """Errors of both GPU forms on configuration 1 (every record, against the FP64 oracle,
+    and O_exact on three records) and of the FP32K form on configuration 4 (six sampled
+    records)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from marfa_amd import synth, lbl
+    out = {}
+    ctx = lbl.LBLContext(0)
+    job = synth.config1()
+    ctx.load_job(job, tips)
+    ref, _ = po.OracleJob(job, tips).pt_tables(po.F64)
+    m = np.abs(ref) >= np.finfo(np.float32).tiny
+    c1 = {}
+    for name, prec in (("fp32k", lbl.FP32K), ("fp64", lbl.FP64)):
+        rec = ctx.pt_tables(lbl.JobSpec.from_job(job, prec)).astype(np.float64)
+        e = np.abs(rec[m] - ref[m]) / np.abs(ref[m])
+        c1[name] = {"max_rel_err": float(e.max()), "rms_rel_err": float(np.sqrt(np.mean(e ** 2)))}
+        if prec == lbl.FP32K:
+            picks = [(0, 0), (4, 5), (9, 9)]
+            c1["fp32k_vs_exact"] = accuracy(job, tips, {p: rec[p] for p in picks}, picks)["vs_exact"]
+    c1["note"] = ("FP32K max: a cascade-limiter near-tie (level 4, record 9, point 2884) that last-bit "
+                  "differences of the grid sums flip (tests/test_oracle.py::test_limiter_near_tie_explains_fp32k_error)")
+    out["config1_all_records"] = c1
+    job = synth.config4()
+    ctx.load_job(job, tips)
+    rec = ctx.pt_tables(lbl.JobSpec.from_job(job, lbl.FP32K))
+    rng = np.random.Generator(np.random.PCG64(4))
+    picks = [(int(l), int(r)) for l, r in zip(rng.integers(0, len(job.atm), 6), rng.integers(0, rec.shape[1], 6))]
+    acc = accuracy(job, tips, {p: rec[p] for p in picks}, picks)
+    out["config4_fp32k"] = {"max_rel_err": acc["max_rel_err"], "rms_rel_err": acc["rms_rel_err"], "records_checked": 6}
+    ctx.close()
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(job, ctx_counts_full, target_samples=1.0e9, max_procs=16):
+    """Time the reference's LBL modules (oracle/_ref/ref_driver) on a bounded sample of the
+    workload and scale to it by profile samples:
+      * one thread: the first, middle and last level over the first subintervals, as many as
+        make about target_samples / 3 profile samples per level (levels differ in cost:
+        Lorentz wings at high pressure, Voigt cores at low pressure);
+      * all host cores: min(max_procs, cores) driver processes at once, each one level
+        (levels evenly spread) over the same subintervals -- the reference's levels are
+        independent and it runs them one after another (README.md:476), so concurrent
+        processes over disjoint levels are its multi-core form.
+    value = the all-cores rate; one_thread holds the single-thread rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
     from marfa_amd import synth, lbl
     if not refrun.available():
         return None
-    atm = synth.Atmosphere(P=job.atm.P[level:level + 1], T=job.atm.T[level:level + 1],
-                           rho=job.atm.rho[level:level + 1], h=job.atm.h[level:level + 1])
-    a = float(job.start_wv)
+    nlev = len(job.atm)
     tips = synth.load_tips()
     ctx = lbl.LBLContext(0)
-    records, s_sub = 25, 0
+
+    nu = np.asarray(job.lines.nu)
+
+    def level_job(lev, a, b):
+        # the lines the sample's windows can read (the reference stops at the first line
+        # beyond the last window), so each process's line file stays small
+        keep = slice(int(np.searchsorted(nu, a - job.cutoff - 20.0)), int(np.searchsorted(nu, b + job.cutoff + 20.0)))
+        atm = synth.Atmosphere(P=job.atm.P[lev:lev + 1], T=job.atm.T[lev:lev + 1], rho=job.atm.rho[lev:lev + 1],
+                               h=job.atm.h[lev:lev + 1])
+        return synth.Job(job.lines.take(keep), atm, a, b, job.cutoff, mol_code=job.mol_code, chi_id=job.chi_id,
+                         shape_id=job.shape_id, target=job.target)
+
+    # subinterval range holding about target/3 samples at one level (samples depend on the
+    # line positions only, so every level of the range has nearly the same count)
+    a, records = float(job.start_wv), 25
+    per = target_samples / 3.0
     while True:
         b = min(float(job.end_wv), a + 10.0 * records)
-        sub = synth.Job(job.lines, atm, a, b, job.cutoff, mol_code=job.mol_code, chi_id=job.chi_id,
-                        shape_id=job.shape_id, target=job.target)
-        # samples of the sample (GPU counter, pinned against the oracle's counters in tests)
-        ctx.load_job(sub, tips)
-        s_sub = ctx.count_samples(lbl.JobSpec.from_job(sub))
-        if s_sub >= target_samples or b >= float(job.end_wv):
+        ctx.load_job(level_job(0, a, b), tips)
+        s_lev = ctx.count_samples(lbl.JobSpec.from_job(level_job(0, a, b)))
+        if s_lev >= per or b >= float(job.end_wv):
             break
-        records = int(records * min(4.0, max(1.25, 1.1 * target_samples / max(s_sub, 1))))
+        records = int(records * min(4.0, max(1.25, 1.1 * per / max(s_lev, 1))))
+    levels1 = sorted({0, nlev // 2, nlev - 1})
+    counts = {}  # samples per level of the range (GPU counter, pinned against the oracle in tests)
+    procs = max(1, min(max_procs, os.cpu_count() or 1, nlev))
+    levelsN = sorted({int(round(x)) for x in np.linspace(0, nlev - 1, procs)})
+    for lev in set(levels1) | set(levelsN):
+        lj = level_job(lev, a, b)
+        ctx.load_job(lj, tips)
+        counts[lev] = ctx.count_samples(lbl.JobSpec.from_job(lj))
     ctx.close()
-    recs, wall, out = refrun.run(sub, use_driver=True)
-    lbl_s = None
-    for line in out.splitlines():
-        if line.startswith("LBL_SECONDS"):
-            lbl_s = float(line.split()[1])
-    pts_sub = recs.shape[0] * recs.shape[1] * lbl.NT
-    t = lbl_s if lbl_s else wall
     samples_full, points_full = ctx_counts_full
-    sps = s_sub / t
-    return {"value": sps * points_full / samples_full, "unit": "pts/s", "cores": 1, "kind": "reference",
-            "sample": (f"reference LBL modules (src/*.f90, amdflang -O2, chi id {job.chi_id}) via oracle/ref_driver: "
-                       f"level {level + 1}, {recs.shape[1]} records {a:.0f}-{b:.0f} cm-1, {t:.1f} s in the "
-                       f"level/subinterval loops ({wall:.1f} s wall); {sps:.3e} samples/s scaled to the full "
-                       f"workload by its samples per point"),
-            "sample_pts_per_s": pts_sub / t, "samples_per_s": sps}
+    # one thread, three levels one after another
+    t1, s1 = 0.0, 0
+    for lev in levels1:
+        _, wall, out = refrun.run(level_job(lev, a, b), use_driver=True)
+        t1 += next(float(l.split()[1]) for l in out.splitlines() if l.startswith("LBL_SECONDS"))
+        s1 += counts[lev]
+    sps1 = s1 / t1
+    # all cores: one process per level, all at once
+    secs, wall = refrun.run_concurrent([level_job(lev, a, b) for lev in levelsN])
+    sN = sum(counts[lev] for lev in levelsN)
+    spsN = sN / wall
+    nrec = len(np.arange(a, b, 10.0))
+    return {"value": spsN * points_full / samples_full, "unit": "pts/s", "cores": len(levelsN), "kind": "reference",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": (f"reference LBL modules (src/*.f90, amdflang -O2, chi id {job.chi_id}) via oracle/ref_driver, "
+                       f"{nrec} subintervals from {a:.0f} cm-1: {len(levelsN)} processes at once, one level each "
+                       f"(levels {levelsN[0] + 1}..{levelsN[-1] + 1} evenly spread), {sN:.3e} samples in {wall:.1f} s wall; "
+                       f"scaled to the full workload by its samples per point"),
+            "samples_per_s": spsN,
+            "one_thread": {"value": sps1 * points_full / samples_full, "unit": "pts/s", "cores": 1,
+                           "samples_per_s": sps1, "seconds": t1, "samples": s1,
+                           "sample": f"levels {[l + 1 for l in levels1]}, the same subintervals, one after another"}}
 
 
 def kernel_ms_forward(model):
@@ -189,7 +286,7 @@ def batch_bench(steps=2, lanes=4, check_jobs=24):
     points = sum(len(j.atm) * b.shapes[i][1] * lbl.NT for i, j in zip(ids, jobs))
     dt = float(np.mean(ms)) / 1e3
     rng = np.random.default_rng(7)
-    errs = []
+    errs, allpts = [], []
     for i in rng.choice(len(jobs), size=min(check_jobs, len(jobs)), replace=False):
         job = jobs[i]
         nlev, nrec, _ = b.shapes[i]
@@ -200,7 +297,8 @@ def batch_bench(steps=2, lanes=4, check_jobs=24):
         ref, _ = po.OracleJob(job, tips).record(po.F64, lev, r)
         m = np.abs(ref) > np.finfo(np.float32).tiny
         if m.any():
-            errs.append(float(np.max(np.abs(rec[m] - ref[m]) / np.abs(ref[m]))))
+            allpts.append(np.abs(rec[m] - ref[m]) / np.abs(ref[m]))
+            errs.append(float(allpts[-1].max()))
     b.close()
     e = np.array(errs)
     return {"workload": "256 jobs: 10^U(3,6) lines, U{10..100} levels (P 90->1e-4 atm, T 735->170 K), "
@@ -211,7 +309,9 @@ def batch_bench(steps=2, lanes=4, check_jobs=24):
                                 for v in range(lbl.NVARIANTS)},
             "setup_s": t1 - t0, "tune_s": t2 - t1,
             "error_spread_vs_fp64": {"jobs_checked": len(errs), "min": float(e.min()), "median": float(np.median(e)),
-                                     "max": float(e.max())} if errs else None}
+                                     "max": float(e.max()), "rms_of_job_max": float(np.sqrt(np.mean(e ** 2))),
+                                     "rms_all_points": float(np.sqrt(np.mean(np.concatenate(allpts) ** 2)))}
+            if errs else None}
 
 
 def large_bench(levels_checked=(0, 49, 99)):
@@ -428,6 +528,11 @@ def main():
         t2 = time.time()
         result["accuracy"] = accuracy(job, tips, recs, picks)
         log(f"[bench] accuracy {result['accuracy']} ({time.time() - t2:.1f}s)")
+
+    if not args.no_accuracy:
+        t2 = time.time()
+        result["config_errors"] = config_errors(tips)
+        log(f"[bench] config errors {result['config_errors']} ({time.time() - t2:.1f}s)")
 
     if not args.no_cpu_baseline:
         t3 = time.time()

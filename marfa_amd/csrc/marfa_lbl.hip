@@ -300,29 +300,53 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
 #pragma unroll
     for (int q = 0; q <= 20; q++) pos[q].zero();
     const int nleft = Ja - J0, nfar = nleft + (J1 - Jb);
+    auto far_j = [&](int v) { return v < nleft ? J0 + v : Jb + (v - nleft); };
+    // The far path reads five fields of a line record (centre, index and the three Lorentz
+    // parameters); those of the next two lines are loaded while the current one is
+    // evaluated.  The whole record is read only by the per-node fallback.
+    struct Far {
+        double nus;
+        int32_t idx;
+        R xs, lorc, gl2;
+    };
+    auto far_at = [&](int v) {
+        Far f;
+        const LineShape<R> &q = shp[far_j(v)];
+        f.nus = q.nus;
+        f.idx = q.idx;
+        f.xs = q.xs;
+        f.lorc = q.lorc;
+        f.gl2 = q.gl2;
+        return f;
+    };
+    Far n1{}, n2{};
+    if ((int)threadIdx.x < nfar) n1 = far_at(threadIdx.x);
+    if ((int)threadIdx.x + BLOCK < nfar) n2 = far_at(threadIdx.x + BLOCK);
     for (int v = threadIdx.x; v < nfar; v += BLOCK) {
-        const int j = v < nleft ? J0 + v : Jb + (v - nleft);
-        const LineShape<R> p = shp[j];
-        if (p.idx < lo || p.idx >= hi) continue;  // the record's window (unshifted centres)
-        const double u = p.nus - org;
-        const bool lw = p.nus < org;
+        const int j = far_j(v);
+        const Far f = n1;
+        n1 = n2;
+        if (v + 2 * BLOCK < nfar) n2 = far_at(v + 2 * BLOCK);
+        if (f.idx < lo || f.idx >= hi) continue;  // the record's window (unshifted centres)
+        const double u = f.nus - org;
+        const bool lw = f.nus < org;
         // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
         const double dmin = lw ? -u : u - 10.0;
         if (lw ? dmin > cut : dmin >= cut) continue;
         // every node, every role: when even the nearest node, |X| >= dmin, is on the Lorentz
         // branch (|X| xs >= margin as shape_eval tests it; rounding is monotone, so every
         // node is), the nodes are summed without per-sample branch tests
-        if (ctl.shape_id == 1 || (R)dmin * p.xs >= margin) {
+        if (ctl.shape_id == 1 || (R)dmin * f.xs >= margin) {
             const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
             const R step = lw ? R(0.5) : R(-0.5);
             // the Lorentz numerator (times chi) is carried as one FP64 factor, so a sample
             // costs the reciprocal, its conversion and one FP64 multiply-add
             if (nochi) {
-                const double lorc = (double)p.lorc;
+                const double lorc = (double)f.lorc;
 #pragma unroll
                 for (int q = 0; q <= 20; q++) {
                     const R ax = q == 0 ? base : fma(step, (R)q, base);
-                    pos[q].add((double)recip<R>(ax * ax + p.gl2) * lorc);
+                    pos[q].add((double)recip<R>(ax * ax + f.gl2) * lorc);
                 }
                 continue;
             }
@@ -331,24 +355,28 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
                 if (b == chi_band<R, CHI>((R)(dmin + 10.0))) {
                     R coef, rate;
                     chi_law<R, CHI>(b, coef, rate);
-                    double lchi = (double)p.lorc * (double)(coef * dexp<R>(-(rate * base)));
+                    double lchi = (double)f.lorc * (double)(coef * dexp<R>(-(rate * base)));
                     const double rho = rho_tab[2 * b + (lw ? 0 : 1)];
 #pragma unroll
                     for (int q = 0; q <= 20; q++) {
                         const R ax = q == 0 ? base : fma(step, (R)q, base);
                         if (q > 0) lchi *= rho;
-                        pos[q].add((double)recip<R>(ax * ax + p.gl2) * lchi);
+                        pos[q].add((double)recip<R>(ax * ax + f.gl2) * lchi);
                     }
                     continue;
                 }
             }
+            LineShape<R> pl;  // chilor_abs reads the Lorentz fields only
+            pl.lorc = f.lorc;
+            pl.gl2 = f.gl2;
 #pragma unroll
             for (int q = 0; q <= 20; q++) {
                 const R ax = q == 0 ? base : fma(step, (R)q, base);
-                pos[q].add((double)chilor_abs<R, CHI>(ax, p, ctl));
+                pos[q].add((double)chilor_abs<R, CHI>(ax, pl, ctl));
             }
             continue;
         }
+        const LineShape<R> p = shp[j];
 #pragma unroll
         for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
     }
@@ -362,8 +390,11 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         aM[c] = pos[2 * c + 1];
         aL[c] = pos[2 * c + 2];
     }
+    LineShape<R> nxt;  // the next centre line's record, loaded while the current one is evaluated
+    if (Ja + (int)threadIdx.x < Jb) nxt = shp[Ja + threadIdx.x];
     for (int j = Ja + threadIdx.x; j < Jb; j += BLOCK) {
-        const LineShape<R> p = shp[j];
+        const LineShape<R> p = nxt;
+        if (j + BLOCK < Jb) nxt = shp[j + BLOCK];
         if (p.idx < lo || p.idx >= hi) continue;
         const double nus = p.nus;
         const double u = nus - org;

@@ -49,6 +49,13 @@ def library() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    try:
+        # PyTorch-ROCm ships its own HIP runtime: load it first, so that the library binds to
+        # the same runtime instead of loading a second one (two runtimes in one process do
+        # not share devices, and torch then finds no GPU)
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i64p, i32p, dp, fp = C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_void_p, C.c_void_p
     L.marfa_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]

@@ -103,6 +103,38 @@ def run(job, use_driver=None, max_levels=None, keep=False, root=None):
             shutil.rmtree(d, ignore_errors=True)
 
 
+def run_concurrent(jobs, root=None):
+    """Run the driver over the reference's modules on every job at once, one process each
+    (the reference's levels are independent, README.md:476), to time the reference on
+    several host cores.  Returns (per-job LBL seconds of process CPU time, wall seconds from
+    the first launch to the last exit)."""
+    dirs, procs = [], []
+    try:
+        cmds = []
+        for job in jobs:
+            d = scratch(job, root)
+            dirs.append(d)
+            os.makedirs(os.path.join(d, "out"))
+            code = SPECIES.get(job.mol_code, ("CO2", "02"))[1]
+            cmds.append(([REF_DRIVER, repr(float(job.start_wv)), repr(float(job.end_wv)), f"data/databases/SYN.{code}",
+                          repr(float(job.cutoff)), "VAC" if job.target else "ACS", "data/Atmospheres/atm.dat",
+                          str(job.chi_id), str(job.mol_code), "out", str(len(job.atm)), str(job.shape_id)], d))
+        t0 = time.perf_counter()
+        for args, d in cmds:
+            procs.append(subprocess.Popen(args, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+        outs = [p.communicate() for p in procs]
+        wall = time.perf_counter() - t0
+        secs = []
+        for p, (out, err) in zip(procs, outs):
+            if p.returncode != 0:
+                raise RuntimeError(f"reference driver failed rc={p.returncode}: {out[-300:]} {err[-300:]}")
+            secs.append(next(float(l.split()[1]) for l in out.splitlines() if l.startswith("LBL_SECONDS")))
+        return secs, wall
+    finally:
+        for d in dirs:
+            shutil.rmtree(d, ignore_errors=True)
+
+
 def probe(commands: str, cwd=None) -> list[str]:
     """Run oracle/_ref/ref_probe on a command script; returns stdout lines."""
     d = tempfile.mkdtemp(prefix="marfa_probe_")
