@@ -464,6 +464,20 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
     return o;
 }
 
+// Key of a lane for a key-segmented lane scan.  Lanes with a value (has) keep their key;
+// a lane without one takes the key of the nearest lane before it that has one (its own
+// values are zero), or -1 before the first.  Along the lanes with values the keys are
+// non-decreasing, but lanes without one can sit between two lanes of one key (a line
+// outside the record's window among the sorted candidates); inheriting the key keeps every
+// key's lanes contiguous, so the scan's segment ends hold each key's sum exactly once.
+__device__ inline int scan_key(bool has, int key) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t before = __ballot(has) & ((1ull << lane) - 1ull);
+    const int from = before ? 63 - __clzll(before) : lane;
+    const int k = __shfl(key, from);
+    return has ? key : (before ? k : -1);
+}
+
 // ---------------------------------------------------------------- accumulators
 // Node and record sums are FP64 in both precision forms: an FP32 sample enters an FP64 sum
 // exactly, which is more accurate than a compensated FP32 pair held in the same two

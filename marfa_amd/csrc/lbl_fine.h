@@ -31,20 +31,6 @@ __device__ inline int bucket_of(double nus, double base, int NB) {
     return (int)x;
 }
 
-// Key of a lane for a key-segmented lane scan.  Lanes with a value (has) keep their key;
-// a lane without one takes the key of the nearest lane before it that has one (its own
-// values are zero), or -1 before the first.  Along the lanes with values the keys are
-// non-decreasing, but lanes without one can sit between two lanes of one key (a line
-// outside the record's window among the sorted candidates); inheriting the key keeps every
-// key's lanes contiguous, so the scan's segment ends hold each key's sum exactly once.
-__device__ inline int scan_key(bool has, int key) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t before = __ballot(has) & ((1ull << lane) - 1ull);
-    const int from = before ? 63 - __clzll(before) : lane;
-    const int k = __shfl(key, from);
-    return has ? key : (before ? k : -1);
-}
-
 template <typename R>
 struct FineCtx {
     const uint64_t *skey;
@@ -617,38 +603,28 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
     }
 }
 
-// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grid k has 2^(k-1)
-// cell pairs in the tile; the whole block works on one grid at a time, L = 256 / 2^(k-1)
-// threads per pair, so that every grid's centre lines (2 cm-1 of them: both walks) are
-// spread over all threads.  Grids 1..3 read their lines from the level's sorted arrays
-// (their groups reach up to 2 cm-1 beyond the tile), grids 4..9 from the staged copy.
-// The L partial sums of a pair are reduced in a fixed butterfly (and, for L > 64, in wave
-// order through red).  Ends with a barrier.  red: 4 x 6 doubles of LDS.
+// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grids 1..3 come
+// from the level-0 kernel (cz13: the record's P, mid, L blocks of grids 1..3), which sums
+// them from the record's centre and edge lines in one pass.  Grid k >= 4 has 2^(k-1) cell
+// pairs in the tile; on grids 4, 5 the whole block works on one grid at a time, L = 256 /
+// 2^(k-1) threads per pair, reading the staged centre lines (grids 4..9 need centre lines
+// within H_3 of the tile); grids 6..9 take one line per lane (level_lanes).  The L partial
+// sums of a pair are reduced in a fixed butterfly.  Ends with a barrier.  red: 4 x 6
+// doubles of LDS.
 template <typename R>
 __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds<R> &lsrc, bool staged, int t,
-                                       double *g, double *red) {
+                                       double *g, double *red, const double *cz13) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // Grids 1..3 read their lines from the sorted records in global memory.  Their 14
-    // candidate ranges (7 pairs x 2 walks) are looked up at once by 14 threads and parked in
-    // grid 9's slots, which nothing uses before the lane pass zeroes them: one wait for the
-    // bucket-index loads instead of one per grid and walk.
-    int *rng = (int *)(g + lvl_acc_off(9));
-    if (tid < 14) {
-        const int k = tid < 2 ? 1 : tid < 6 ? 2 : 3, e = tid - (k == 1 ? 0 : k == 2 ? 2 : 6);
-        const int npair = 1 << (k - 1), gl = t * npair + 1 + (e >> 1), gw = (e & 1) ? (5 << k) + 1 - gl : gl;
-        const double Hk = hk(k);
-        int j0, j1;
-        if (e & 1) gsrc.template group_range<true>((double)(2 * gw) * Hk, (double)(2 * gw + 2) * Hk, j0, j1);
-        else gsrc.template group_range<false>((double)(2 * gw) * Hk, (double)(2 * gw + 2) * Hk, j0, j1);
-        rng[2 * tid] = j0;
-        rng[2 * tid + 1] = j1;
+    // grids 1..3 of the tile: cells [t 2^k, (t + 1) 2^k) of grid k, 3 x (2 + 4 + 8) values
+    if (tid < 42) {
+        const int k = tid < 6 ? 1 : tid < 18 ? 2 : 3, e = tid - (k == 1 ? 0 : k == 2 ? 6 : 18);
+        const int nk = 1 << k, role = e / nk, c = e % nk;
+        g[lvl_acc_off(k) + role * nk + c] = cz13[30 * ((1 << k) - 2) + role * (10 << k) + t * nk + c];
     }
-    __syncthreads();
 #pragma unroll 1
-    for (int k = 1; k <= 9; k++) {
+    for (int k = 4; k <= 9; k++) {
         if (k >= LANES_K0) {
             if (k == LANES_K0) {
-                __syncthreads();  // every wavefront is past grids 1..3 (their ranges in grid 9's slots)
                 if (staged) level_lanes<R>(lsrc, t, g);
                 else level_lanes<R>(gsrc, t, g);
             }
@@ -680,13 +656,9 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
         double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         // the record's edge pairs also take wing lines, which are not staged
         const bool edge = (t == 0 && pi == 0) || (t == TILES - 1 && pi == npair - 1);
-        if (k >= 4 && staged && !edge) {
+        if (staged && !edge) {
             pair_walk<R, false>(lsrc, k, gl, sl, L, s);
             pair_walk<R, true>(lsrc, k, gr, sl, L, s);
-        } else if (k <= 3) {
-            const int *r = rng + 2 * (2 * ((1 << (k - 1)) - 1) + 2 * pi);
-            pair_walk<R, false>(gsrc, k, gl, sl, L, s, r);
-            pair_walk<R, true>(gsrc, k, gr, sl, L, s, r + 2);
         } else {
             pair_walk<R, false>(gsrc, k, gl, sl, L, s);
             pair_walk<R, true>(gsrc, k, gr, sl, L, s);
@@ -777,9 +749,8 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
     const GlobalSrc<R> gsrc(f);
     CentreLds<R> lsrc;
     const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc, cap);
-    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red);
-
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
+    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red, cz + NG0);
     if (a.grids && r == 0) {
         for (int e = tid; e < NACC; e += BLOCK) {
             int k = 1;
@@ -894,8 +865,8 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     const GlobalSrc<R> gsrc(f);
     CentreLds<R> lsrc;
     const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
-    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red);
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
+    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red, cz + NG0);
     tile_cascade(g, cz, t);
 
     const double mul = (double)a.outmul[a.lev0 + lev] / sc;

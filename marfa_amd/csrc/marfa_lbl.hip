@@ -41,7 +41,9 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int TILES = 10;      // 1 cm-1 tiles per 10 cm-1 record
-constexpr int NCOARSE = 32;    // RK1, P[10], M[10], L[10], pad
+constexpr int NG0 = 32;        // grid 0 of a record: RK1, P[10], M[10], L[10], pad
+constexpr int NG13 = 420;      // grids 1..3 of a record: P, mid, L blocks of 20, 40, 80 cells
+constexpr int NCOARSE = NG0 + NG13;  // per record: the level-0 kernel's sums
 constexpr int NACC = 3066;     // per tile: 3*(2+4+...+512) nodes of grids 1..9
 
 enum : uint8_t { CLS_NONE = 0, CLS_LEFT = 1, CLS_CENTER = 2, CLS_RIGHT = 3 };
@@ -239,6 +241,99 @@ __device__ inline void chi_law(int b, R &coef, R &rate) {
     }
 }
 
+// Grids 1..3 of a record from one line per lane (src/LineGridCalc.f90: the walks of
+// centerLBL on levels 1..3, and the cells next to the record's edges that leftLBL /
+// rightLBL fill on levels k <= m).  For a centre line at u = nu* - org the left walk on grid
+// k covers cells 2g-1 .. n-1, n = floor(u 2^k), g = n >> 1: 3 positions (n even) or 5 (n
+// odd) of the pair (2g-1, 2g) from X = u - (4g - 4 + e) H_k/2; the right walk is the mirror
+// image in v = 10 - u (pair NT_k/2 + 1 - g from the left).  A left wing with d = -u < H_{k-1}
+// fills pair 1 from X = u + (e - 4) H_k/2 (5 positions when d >= H_k), a right wing with
+// d = u - 10 < H_{k-1} the last pair from X = d + (4 - e) H_k/2.  Along lanes holding lines in
+// sorted order, the pair index of each (grid, walk) does not decrease, so a key-segmented
+// lane scan sums each pair's values and the last lane of a segment adds them to the pair's
+// six slots in gw (P, mid, L blocks of 10 2^k cells per grid, grid 1 first).  Every lane of
+// the wavefront calls this (the scans); ok: the lane holds a line of the record's window
+// that deposits.
+template <typename R, int CHI>
+__device__ inline void grids13_lanes(bool ok, double nus, double org, const LineShape<R> &p, const ShapeCtl &ctl,
+                                     double *gw) {
+    const int lane = threadIdx.x & 63;
+    const double u = nus - org;
+    // classes as the deposit dispatch takes them (src/LBL.f90:81-98 on the shifted centre)
+    const bool left = ok && nus < org, right = ok && nus > org + 10.0, centre = ok && !left && !right && u < 10.0;
+#pragma unroll 1
+    for (int k = 1; k <= 3; k++) {
+        const double Hk = hk(k), hq = 0.5 * Hk;
+        const int ntk = 10 << k, npair = ntk >> 1, base = 30 * ((1 << k) - 2);
+#pragma unroll 1
+        for (int side = 0; side < 2; side++) {  // 0: left walk (and right wings), 1: right walk (left wings)
+            int key = -1;
+            bool five = false, plus = false;
+            double B = 0.0, C = 0.0;
+            if (centre) {
+                const double w = side ? 10.0 - u : u;
+                const int nn = (int)floor(__builtin_ldexp(w, k));
+                const int gi = nn >> 1;
+                if (gi >= 1) {
+                    key = side ? npair - gi : gi - 1;
+                    five = nn & 1;
+                    B = w;
+                    C = (double)(4 * gi - 4);
+                }
+            } else if (side == 0 && right && u - 10.0 < 2.0 * Hk) {
+                key = npair - 1;
+                five = u - 10.0 >= Hk;
+                B = u - 10.0;
+                C = -4.0;
+            } else if (side == 1 && left && -u < 2.0 * Hk) {
+                key = 0;
+                five = -u >= Hk;
+                B = u;
+                C = -4.0;
+                plus = true;
+            }
+            R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // e = 0..4 and e = 2 once more (n odd)
+            if (key >= 0) {
+#pragma unroll 1
+                for (int e = 0; e < (five ? 5 : 3); e++) {
+                    const double m = (C + (double)e) * hq;
+                    const R x = shape_eval<R, false, CHI>(plus ? B + m : B - m, p, ctl);
+                    if (e == 0) v[0] = x;
+                    else if (e == 1) v[1] = x;
+                    else if (e == 2) {
+                        v[2] = x;
+                        if (five) v[5] = x;
+                    } else if (e == 3) v[3] = x;
+                    else v[4] = x;
+                }
+            }
+            // slots P1 M1 L1 P2 M2 L2 of the pair (right walk: the mirror image)
+            R s0 = side ? v[4] : v[0], s1 = side ? v[3] : v[1], s2 = side ? v[5] : v[2];
+            R s3 = side ? v[2] : v[5], s4 = side ? v[1] : v[3], s5 = side ? v[0] : v[4];
+            key = scan_key(key >= 0, key);
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int kp = __shfl_up(key, off);
+                const R t0 = __shfl_up(s0, off), t1 = __shfl_up(s1, off), t2 = __shfl_up(s2, off);
+                const R t3 = __shfl_up(s3, off), t4 = __shfl_up(s4, off), t5 = __shfl_up(s5, off);
+                if (lane >= off && kp == key) {
+                    s0 += t0; s1 += t1; s2 += t2; s3 += t3; s4 += t4; s5 += t5;
+                }
+            }
+            const int kn = __shfl_down(key, 1);
+            if (key >= 0 && (lane == 63 || kn != key)) {
+                const int c = 2 * key;  // 0-based cells c, c + 1 of grid k
+                gw[base + c] += (double)s0;
+                gw[base + ntk + c] += (double)s1;
+                gw[base + 2 * ntk + c] += (double)s2;
+                gw[base + c + 1] += (double)s3;
+                gw[base + ntk + c + 1] += (double)s4;
+                gw[base + 2 * ntk + c + 1] += (double)s5;
+            }
+        }
+    }
+}
+
 // One workgroup per record.  Threads take the level's sorted line records (shifted centre,
 // line index and the precomputed profile parameters) in order, so the 64 lines of a
 // wavefront lie within a fraction of a cm-1 and take the same branches.  A line at least
@@ -390,35 +485,46 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         aM[c] = pos[2 * c + 1];
         aL[c] = pos[2 * c + 2];
     }
+    // Grids 1..3 of the record come from the same near lines (centre lines and the wings
+    // within 1 cm-1 of an edge): per wavefront, partial node sums in LDS, added to by the
+    // last lane of each pair's segment after a lane scan (grids13_lanes); the wavefronts'
+    // partials are summed in wave order at the end.
+    __shared__ double g13[BLOCK / 64][NG13];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int e = lane; e < NG13; e += 64) g13[wv][e] = 0.0;
     LineShape<R> nxt;  // the next centre line's record, loaded while the current one is evaluated
     if (Ja + (int)threadIdx.x < Jb) nxt = shp[Ja + threadIdx.x];
-    for (int j = Ja + threadIdx.x; j < Jb; j += BLOCK) {
+    // the wavefront's lanes take consecutive lines, all lanes every pass (the scans)
+    for (int c0 = Ja + 64 * wv; c0 < Jb; c0 += BLOCK) {
+        const int j = c0 + lane;
         const LineShape<R> p = nxt;
         if (j + BLOCK < Jb) nxt = shp[j + BLOCK];
-        if (p.idx < lo || p.idx >= hi) continue;
+        bool ok = j < Jb && !(p.idx < lo || p.idx >= hi);
         const double nus = p.nus;
         const double u = nus - org;
-        uint32_t mP, mM, mL;
-        if (!level0_masks(nus, org, cut, mP, mM, mL)) continue;
-        const uint32_t need = mP | mM | mL | 1u;
+        uint32_t mP = 0, mM = 0, mL = 0;
+        if (ok) ok = level0_masks(nus, org, cut, mP, mM, mL);
+        if (ok) {
+            const uint32_t need = mP | mM | mL | 1u;
 #pragma unroll
-        for (int q = 0; q <= 20; q++) {
-            // positions no role needs are moved far out, onto the cheap Lorentz branch
-            const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
-            const double val = (double)shape_eval<R, false, CHI>(X, p, ctl);
-            if (q == 0) aR.add(val);
-            if (q & 1) {
-                aM[q >> 1].add(((mM >> q) & 1u) ? val : 0.0);
-            } else {
-                if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : 0.0);
-                if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : 0.0);
+            for (int q = 0; q <= 20; q++) {
+                // positions no role needs are moved far out, onto the cheap Lorentz branch
+                const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
+                const double val = (double)shape_eval<R, false, CHI>(X, p, ctl);
+                if (q == 0) aR.add(val);
+                if (q & 1) {
+                    aM[q >> 1].add(((mM >> q) & 1u) ? val : 0.0);
+                } else {
+                    if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : 0.0);
+                    if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : 0.0);
+                }
             }
         }
+        grids13_lanes<R, CHI>(ok, nus, org, p, ctl, g13[wv]);
     }
 
     // fixed-order reduction: wave butterflies, then the 4 waves in order, in FP64
     __shared__ double red[BLOCK / 64][31];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double vals[31];
     vals[0] = aR.value();
 #pragma unroll
@@ -438,6 +544,11 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         double x = red[0][threadIdx.x];
         for (int w = 1; w < BLOCK / 64; w++) x += red[w][threadIdx.x];
         a.coarse[(int64_t)r * NCOARSE + threadIdx.x] = x;
+    }
+    for (int e = threadIdx.x; e < NG13; e += BLOCK) {
+        double x = g13[0][e];
+        for (int w = 1; w < BLOCK / 64; w++) x += g13[w][e];
+        a.coarse[(int64_t)r * NCOARSE + NG0 + e] = x;
     }
 }
 
