@@ -603,26 +603,28 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
     }
 }
 
-// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grids 1..3 come
-// from the level-0 kernel (cz13: the record's P, mid, L blocks of grids 1..3), which sums
-// them from the record's centre and edge lines in one pass.  Grid k >= 4 has 2^(k-1) cell
-// pairs in the tile; on grids 4, 5 the whole block works on one grid at a time, L = 256 /
-// 2^(k-1) threads per pair, reading the staged centre lines (grids 4..9 need centre lines
-// within H_3 of the tile); grids 6..9 take one line per lane (level_lanes).  The L partial
-// sums of a pair are reduced in a fixed butterfly.  Ends with a barrier.  red: 4 x 6
-// doubles of LDS.
+// Pre-cascade sums of grids 1..9 at the nodes of tile t, into g (LDS).  Grids 1..5 come
+// from the level-0 kernel (cz15: the record's P, mid, L blocks of grids 1..5), which sums
+// them from the record's centre and edge lines in one pass.  Grids 6..9 take one staged
+// line per lane (level_lanes: their centre lines lie within H_3 of the tile), the edge
+// pairs' wing lines a walk over global memory.  Ends with a barrier.
 template <typename R>
 __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds<R> &lsrc, bool staged, int t,
-                                       double *g, double *red, const double *cz13) {
+                                       double *g, double *red, const double *cz15) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // grids 1..3 of the tile: cells [t 2^k, (t + 1) 2^k) of grid k, 3 x (2 + 4 + 8) values
-    if (tid < 42) {
-        const int k = tid < 6 ? 1 : tid < 18 ? 2 : 3, e = tid - (k == 1 ? 0 : k == 2 ? 6 : 18);
-        const int nk = 1 << k, role = e / nk, c = e % nk;
-        g[lvl_acc_off(k) + role * nk + c] = cz13[30 * ((1 << k) - 2) + role * (10 << k) + t * nk + c];
+    // grids 1..5 of the tile: cells [t 2^k, (t + 1) 2^k) of grid k, 3 x (2 + 4 + ... + 32)
+    // values: thread e < 186 copies value e
+    for (int e = tid; e < 186; e += BLOCK) {
+        int k = 1, o = e;
+        while (o >= 3 << k) {
+            o -= 3 << k;
+            k++;
+        }
+        const int nk = 1 << k, role = o / nk, c = o % nk;
+        g[lvl_acc_off(k) + role * nk + c] = cz15[30 * ((1 << k) - 2) + role * (10 << k) + t * nk + c];
     }
 #pragma unroll 1
-    for (int k = 4; k <= 9; k++) {
+    for (int k = LANES_K0; k <= 9; k++) {
         if (k >= LANES_K0) {
             if (k == LANES_K0) {
                 if (staged) level_lanes<R>(lsrc, t, g);
@@ -646,50 +648,6 @@ __device__ inline void tile_level_sums(const GlobalSrc<R> &gsrc, const CentreLds
                         g[accb + 2 * nk + c1 + c] += s[3 * c + 2];
                     }
                 }
-            }
-            continue;
-        }
-        const int npair = 1 << (k - 1), L = BLOCK >> (k - 1);
-        const int pi = tid / L, sl = tid % L;
-        const int gl = t * npair + 1 + pi;         // left-walk group of the pair
-        const int gr = (5 << k) + 1 - gl;          // right-walk group (NT_k / 2 + 1 - gl)
-        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        // the record's edge pairs also take wing lines, which are not staged
-        const bool edge = (t == 0 && pi == 0) || (t == TILES - 1 && pi == npair - 1);
-        if (staged && !edge) {
-            pair_walk<R, false>(lsrc, k, gl, sl, L, s);
-            pair_walk<R, true>(lsrc, k, gr, sl, L, s);
-        } else {
-            pair_walk<R, false>(gsrc, k, gl, sl, L, s);
-            pair_walk<R, true>(gsrc, k, gr, sl, L, s);
-        }
-        for (int off = (L < 64 ? L : 64) >> 1; off > 0; off >>= 1)
-#pragma unroll
-            for (int c = 0; c < 6; c++) s[c] += __shfl_xor(s[c], off);
-        const int nk = 1 << k, accb = lvl_acc_off(k);
-        if (L > 64) {  // pairs spanning several waves: wave partials in wave order
-            if (lane == 0)
-#pragma unroll
-                for (int c = 0; c < 6; c++) red[wv * 6 + c] = s[c];
-            __syncthreads();
-            if (tid < npair) {
-                const int w0 = tid * (L >> 6), w1 = w0 + (L >> 6);
-#pragma unroll
-                for (int c = 0; c < 6; c++) {
-                    double x = red[w0 * 6 + c];
-                    for (int w = w0 + 1; w < w1; w++) x += red[w * 6 + c];
-                    s[c] = x;
-                }
-            }
-            __syncthreads();
-        }
-        if (L > 64 ? tid < npair : sl == 0) {
-            const int c1 = 2 * (L > 64 ? tid : pi);  // tile cells 2pi+1, 2pi+2 (0-based 2pi, 2pi+1)
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                g[accb + c1 + c] = s[3 * c];
-                g[accb + nk + c1 + c] = s[3 * c + 1];
-                g[accb + 2 * nk + c1 + c] = s[3 * c + 2];
             }
         }
     }

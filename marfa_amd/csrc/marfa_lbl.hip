@@ -43,7 +43,9 @@ constexpr int BLOCK = 256;
 constexpr int TILES = 10;      // 1 cm-1 tiles per 10 cm-1 record
 constexpr int NG0 = 32;        // grid 0 of a record: RK1, P[10], M[10], L[10], pad
 constexpr int NG13 = 420;      // grids 1..3 of a record: P, mid, L blocks of 20, 40, 80 cells
-constexpr int NCOARSE = NG0 + NG13;  // per record: the level-0 kernel's sums
+constexpr int NG45 = 1440;     // grids 4, 5 of a record: P, mid, L blocks of 160, 320 cells
+constexpr int KNEAR = 5;       // grids 1..KNEAR are summed by the level-0 kernel
+constexpr int NCOARSE = NG0 + NG13 + NG45;  // per record: the level-0 kernel's sums
 constexpr int NACC = 3066;     // per tile: 3*(2+4+...+512) nodes of grids 1..9
 
 enum : uint8_t { CLS_NONE = 0, CLS_LEFT = 1, CLS_CENTER = 2, CLS_RIGHT = 3 };
@@ -241,8 +243,8 @@ __device__ inline void chi_law(int b, R &coef, R &rate) {
     }
 }
 
-// Grids 1..3 of a record from one line per lane (src/LineGridCalc.f90: the walks of
-// centerLBL on levels 1..3, and the cells next to the record's edges that leftLBL /
+// Grids 1..5 of a record from one line per lane (src/LineGridCalc.f90: the walks of
+// centerLBL on levels 1..5, and the cells next to the record's edges that leftLBL /
 // rightLBL fill on levels k <= m).  For a centre line at u = nu* - org the left walk on grid
 // k covers cells 2g-1 .. n-1, n = floor(u 2^k), g = n >> 1: 3 positions (n even) or 5 (n
 // odd) of the pair (2g-1, 2g) from X = u - (4g - 4 + e) H_k/2; the right walk is the mirror
@@ -251,20 +253,21 @@ __device__ inline void chi_law(int b, R &coef, R &rate) {
 // d = u - 10 < H_{k-1} the last pair from X = d + (4 - e) H_k/2.  Along lanes holding lines in
 // sorted order, the pair index of each (grid, walk) does not decrease, so a key-segmented
 // lane scan sums each pair's values and the last lane of a segment adds them to the pair's
-// six slots in gw (P, mid, L blocks of 10 2^k cells per grid, grid 1 first).  Every lane of
-// the wavefront calls this (the scans); ok: the lane holds a line of the record's window
+// six slots (P, mid, L blocks of 10 2^k cells per grid): grids 1..3 in g13 (FP64), grids 4, 5
+// in g45 (precision R: a slot there takes the segments of a few chunks at most).  Every lane
+// of the wavefront calls this (the scans); ok: the lane holds a line of the record's window
 // that deposits.
 template <typename R, int CHI>
-__device__ inline void grids13_lanes(bool ok, double nus, double org, const LineShape<R> &p, const ShapeCtl &ctl,
-                                     double *gw) {
+__device__ inline void near_grids_lanes(bool ok, double nus, double org, const LineShape<R> &p, const ShapeCtl &ctl,
+                                        double *g13, R *g45) {
     const int lane = threadIdx.x & 63;
     const double u = nus - org;
     // classes as the deposit dispatch takes them (src/LBL.f90:81-98 on the shifted centre)
     const bool left = ok && nus < org, right = ok && nus > org + 10.0, centre = ok && !left && !right && u < 10.0;
 #pragma unroll 1
-    for (int k = 1; k <= 3; k++) {
+    for (int k = 1; k <= KNEAR; k++) {
         const double Hk = hk(k), hq = 0.5 * Hk;
-        const int ntk = 10 << k, npair = ntk >> 1, base = 30 * ((1 << k) - 2);
+        const int ntk = 10 << k, npair = ntk >> 1, base = 30 * ((1 << k) - 2) - (k > 3 ? NG13 : 0);
 #pragma unroll 1
         for (int side = 0; side < 2; side++) {  // 0: left walk (and right wings), 1: right walk (left wings)
             int key = -1;
@@ -323,12 +326,21 @@ __device__ inline void grids13_lanes(bool ok, double nus, double org, const Line
             const int kn = __shfl_down(key, 1);
             if (key >= 0 && (lane == 63 || kn != key)) {
                 const int c = 2 * key;  // 0-based cells c, c + 1 of grid k
-                gw[base + c] += (double)s0;
-                gw[base + ntk + c] += (double)s1;
-                gw[base + 2 * ntk + c] += (double)s2;
-                gw[base + c + 1] += (double)s3;
-                gw[base + ntk + c + 1] += (double)s4;
-                gw[base + 2 * ntk + c + 1] += (double)s5;
+                if (k <= 3) {
+                    g13[base + c] += (double)s0;
+                    g13[base + ntk + c] += (double)s1;
+                    g13[base + 2 * ntk + c] += (double)s2;
+                    g13[base + c + 1] += (double)s3;
+                    g13[base + ntk + c + 1] += (double)s4;
+                    g13[base + 2 * ntk + c + 1] += (double)s5;
+                } else {
+                    g45[base + c] += s0;
+                    g45[base + ntk + c] += s1;
+                    g45[base + 2 * ntk + c] += s2;
+                    g45[base + c + 1] += s3;
+                    g45[base + ntk + c + 1] += s4;
+                    g45[base + 2 * ntk + c + 1] += s5;
+                }
             }
         }
     }
@@ -485,13 +497,15 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         aM[c] = pos[2 * c + 1];
         aL[c] = pos[2 * c + 2];
     }
-    // Grids 1..3 of the record come from the same near lines (centre lines and the wings
+    // Grids 1..5 of the record come from the same near lines (centre lines and the wings
     // within 1 cm-1 of an edge): per wavefront, partial node sums in LDS, added to by the
-    // last lane of each pair's segment after a lane scan (grids13_lanes); the wavefronts'
-    // partials are summed in wave order at the end.
+    // last lane of each pair's segment after a lane scan (near_grids_lanes); the
+    // wavefronts' partials are summed in wave order at the end.
     __shared__ double g13[BLOCK / 64][NG13];
+    __shared__ R g45[BLOCK / 64][NG45];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int e = lane; e < NG13; e += 64) g13[wv][e] = 0.0;
+    for (int e = lane; e < NG45; e += 64) g45[wv][e] = R(0);
     LineShape<R> nxt;  // the next centre line's record, loaded while the current one is evaluated
     if (Ja + (int)threadIdx.x < Jb) nxt = shp[Ja + threadIdx.x];
     // the wavefront's lanes take consecutive lines, all lanes every pass (the scans)
@@ -520,7 +534,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
                 }
             }
         }
-        grids13_lanes<R, CHI>(ok, nus, org, p, ctl, g13[wv]);
+        near_grids_lanes<R, CHI>(ok, nus, org, p, ctl, g13[wv], g45[wv]);
     }
 
     // fixed-order reduction: wave butterflies, then the 4 waves in order, in FP64
@@ -549,6 +563,11 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         double x = g13[0][e];
         for (int w = 1; w < BLOCK / 64; w++) x += g13[w][e];
         a.coarse[(int64_t)r * NCOARSE + NG0 + e] = x;
+    }
+    for (int e = threadIdx.x; e < NG45; e += BLOCK) {
+        double x = (double)g45[0][e];
+        for (int w = 1; w < BLOCK / 64; w++) x += (double)g45[w][e];
+        a.coarse[(int64_t)r * NCOARSE + NG0 + NG13 + e] = x;
     }
 }
 
