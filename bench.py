@@ -433,7 +433,7 @@ def main():
     nlev = len(job.atm)
     points = nlev * nrec * lbl.NT
     samples, coarse_evals = ctx.count_samples(spec, coarse=True)
-    log(f"[bench] {nrec} records/level, {points:.4e} points, {samples:.4e} samples ({coarse_evals:.4e} level-0)")
+    log(f"[bench] {nrec} records/level, {points:.4e} points, {samples:.4e} samples ({coarse_evals:.4e} in the level-0 kernel)")
 
     out = torch.empty((nlev, nrec, lbl.NT), dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
@@ -459,10 +459,11 @@ def main():
     records = nlev * nrec
     kern = {
         "coarse_kernel": {"ms": phase[2], "flop": FLOP_PER_SAMPLE * coarse_evals,
-                          "flop_model": f"{FLOP_PER_SAMPLE} FLOP x {coarse_evals} grid-0 samples"},
+                          "flop_model": f"{FLOP_PER_SAMPLE} FLOP x {coarse_evals} samples (grid-0 evaluations "
+                                        f"and grids 1-5)"},
         "fine_kernel": {"ms": phase[3],
                         "flop": FLOP_PER_SAMPLE * (samples - coarse_evals) + FLOP_PER_RECORD_CASCADE * records,
-                        "flop_model": f"{FLOP_PER_SAMPLE} FLOP x {samples - coarse_evals} grid-1..9/fine samples"
+                        "flop_model": f"{FLOP_PER_SAMPLE} FLOP x {samples - coarse_evals} grid-6..9/fine-point samples"
                                       f" + {FLOP_PER_RECORD_CASCADE} FLOP x {records} record cascades"},
     }
     for k, v in kern.items():

@@ -197,8 +197,9 @@ int marfa_batch_records(marfa_batch *b, int32_t job, float *out_host);
 /* Work counts of the job, summed over levels and subintervals (computed on the device):
  * samples[0] = exact count of distinct profile evaluations (samples) of the reference's
  * grid geometry (21 + 2m per wing line, the walks and core points of a centre line);
- * samples[1] = the part of samples[0] on grid 0 (the 21 half-integer positions and RK(1)),
- * which the level-0 deposit kernel evaluates; the rest belongs to the fine-grid kernel. */
+ * samples[1] = the part the level-0 deposit kernel evaluates: its grid-0 evaluations (the 21
+ * half-integer positions and RK(1)) and the samples of grids 1..5; the rest (grids 6..9 and
+ * the fine points) belongs to the fine-grid kernel. */
 int marfa_count_samples(marfa_ctx *ctx, const marfa_job *job, int64_t samples[2]);
 
 /* Pre-cascade grid sums of one record (level, subinterval index), in the reference's

@@ -596,7 +596,8 @@ void launch_fine(int variant, dim3 g, hipStream_t st, const KArgs &ka) {
 // ------------------------------------------------------------------ sample counts
 __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long long *counts) {
     // counts[2r] = distinct samples of the reference's geometry in record r,
-    // counts[2r+1] = evaluations made by coarse_kernel for record r
+    // counts[2r+1] = the part of them the level-0 kernel evaluates: its grid-0 evaluations
+    // and the samples of grids 1..KNEAR
     const int nitems = a.nlev * a.nrec;
     const int r = blockIdx.x;
     if (r >= nitems) return;
@@ -615,12 +616,14 @@ __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long lon
             int m = 0;
             while (m <= 9 && d < hk(m)) m++;
             cnt += 21 + 2 * m;
+            c0 += 2 * min(m, KNEAR);  // the near grids' cells next to the edge
         } else if (nus > org + 10.0) {
             const double dr = u - 10.0;
             if (dr >= cut) continue;
             int m = 0;
             while (m <= 9 && dr < hk(m)) m++;
             cnt += 21 + 2 * m;
+            c0 += 2 * min(m, KNEAR);
         } else if (u < 10.0) {
             const double v = 10.0 - u;
             long long cells = 0;
@@ -629,6 +632,7 @@ __global__ __launch_bounds__(BLOCK) void count_kernel(KArgs a, unsigned long lon
                 const int e = u >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(u, k)) - 1 : 0;
                 const int er = v >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(v, k)) - 1 : 0;
                 cells += (e - 2 * ep) + (er - 2 * epr);
+                if (k >= 1 && k <= KNEAR) c0 += 2 * ((e - 2 * ep) + (er - 2 * epr));
                 ep = e;
                 epr = er;
             }
