@@ -162,6 +162,26 @@ int marfa_pt_tables_adjoint_dev(marfa_ctx *ctx, const marfa_job *job, const floa
 int marfa_update_line_params(marfa_ctx *ctx, const float *S, const float *gair, const float *nT,
                              const float *delta, void *stream);
 
+/* The spectral loss of the differentiable form, L = sum_i w_i (ln k_i - ln khat_i)^2 over the
+ * points where k_i > 0 and khat_i > 0 (FP64 logarithms and sum, in a fixed order), and its
+ * gradient dL/dk_i = 2 w_i (ln k_i - ln khat_i) / k_i (0 elsewhere).  No counterpart in the
+ * reference: the loss of the gradient workloads, fused so that a backward pass reads the
+ * records once.  k, khat: n FP32 values in device memory (w: FP32 weights, or NULL for unit ones);
+ * every weight is further multiplied by wscale.  marfa_log_loss writes one double to
+ * loss_dev, using partial_dev (MARFA_LOSS_PARTIALS doubles, device) for the block sums;
+ * marfa_log_loss_grad writes n FP32 values to grad_dev, each times *gscale_dev (device, or
+ * NULL for 1).  Both are enqueued on stream (NULL: the default stream) and allocate nothing. */
+#define MARFA_LOSS_PARTIALS 2048
+int marfa_log_loss(int64_t n, const float *k, const float *khat, const float *w, double wscale, double *partial_dev,
+                   double *loss_dev, void *stream);
+int marfa_log_loss_grad(int64_t n, const float *k, const float *khat, const float *w, double wscale,
+                        const double *gscale_dev, float *grad_dev, void *stream);
+/* the same for FP64 records (k, khat, grad_dev in FP64; weights FP32) */
+int marfa_log_loss_f64(int64_t n, const double *k, const double *khat, const float *w, double wscale,
+                       double *partial_dev, double *loss_dev, void *stream);
+int marfa_log_loss_grad_f64(int64_t n, const double *k, const double *khat, const float *w, double wscale,
+                            const double *gscale_dev, double *grad_dev, void *stream);
+
 /* The context's line count, level count and device (any pointer may be NULL). */
 int marfa_ctx_info(const marfa_ctx *ctx, int64_t *nlines, int32_t *nlev, int32_t *device);
 

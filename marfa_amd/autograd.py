@@ -89,8 +89,29 @@ class PTTables(torch.autograd.Function):
         return tuple(g.to(device=t.device, dtype=t.dtype) for g, t in zip(outs, saved)) + (None,)
 
 
+class SpectralLoss(torch.autograd.Function):
+    """sum w (ln k - ln k_hat)^2 by torch.ops.marfa.log_loss (one pass over the points, FP64
+    logarithms and a fixed-order sum); backward: dL/dk by torch.ops.marfa.log_loss_grad."""
+
+    @staticmethod
+    def forward(ctx, records, target, weight, wscale):
+        ctx.save_for_backward(records, target, weight)
+        ctx.wscale = wscale
+        return ops.load().log_loss(records, target, weight, wscale)
+
+    @staticmethod
+    def backward(ctx, g):
+        records, target, weight = ctx.saved_tensors
+        grad = ops.load().log_loss_grad(records, target, weight, ctx.wscale, g)
+        return grad, None, None, None
+
+
 def spectral_loss(records, target, weight=1.0):
-    """sum w (log k - log k_hat)^2 over the points where both spectra are positive."""
-    m = (records > 0) & (target > 0)
-    d = torch.where(m, torch.log(torch.where(m, records, 1.0)) - torch.log(torch.where(m, target, 1.0)), 0.0)
-    return (weight * d.double() ** 2).sum()
+    """sum w (log k - log k_hat)^2 over the points where both spectra are positive (a float64
+    scalar).  records: float32 or float64 CUDA tensor; target: the same shape (cast to the
+    records' dtype); weight: a number or a tensor of the records' shape."""
+    target = target.to(device=records.device, dtype=records.dtype).contiguous()
+    if isinstance(weight, torch.Tensor):
+        w = weight.to(device=records.device, dtype=torch.float32).expand_as(records).contiguous()
+        return SpectralLoss.apply(records.contiguous(), target, w, 1.0)
+    return SpectralLoss.apply(records.contiguous(), target, None, float(weight))

@@ -19,6 +19,7 @@ struct AdjArgs {
     const float *P;                                  // [nlev] absolute level index
     const double *pself;                             // [nlev]
     const float *gbar;                               // [batch records][30690 + NT]
+    const float *gpos;                               // [batch records][GPOS]: adjoint_gpos_kernel
     double *gline;                                   // [batch levels][nl][4]: S, gair, nT, delta
     double *glev;                                    // [batch levels][blocks][2]: T, P
 };
@@ -28,9 +29,34 @@ __device__ inline int node_m(int k, int I) { return 30 * ((1 << k) - 1) + (10 <<
 __device__ inline int node_l(int k, int I) { return 30 * ((1 << k) - 1) + 2 * (10 << k) + (I - 1); }
 __device__ inline int node_rk(int i) { return 30690 + i - 1; }
 
+// Per record, the node adjoints of the 21 positions q/2 cm-1 of grid 0 with every role at a
+// position summed (q even: L of cell q/2 and P of cell q/2 + 1, and RK(1) at q = 0; q odd:
+// the mid of cell (q + 1)/2): a wing line 1 cm-1 or more outside the subinterval samples
+// every position in every role (src/LineGridCalc.f90:21-33, :508-518), so this is its weight.
+constexpr int GPOS = 24;  // 21 positions, padded
+
+__global__ __launch_bounds__(BLOCK) void adjoint_gpos_kernel(const float *gbar, int nrecs, float *gpos) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nrecs * GPOS) return;
+    const int r = i / GPOS, q = i % GPOS;
+    const float *gr = gbar + (int64_t)r * (30690 + NT);
+    float g = 0.f;
+    if (q <= 20) {
+        if (q & 1) {
+            g = gr[10 + (q >> 1)];
+        } else {
+            if (q > 0) g += gr[20 + (q >> 1) - 1];
+            if (q < 20) g += gr[q >> 1];
+            if (q == 0) g += gr[30690];
+        }
+    }
+    gpos[i] = g;
+}
+
 template <typename R>
 struct LineAdj {
     const float *gr;  // dL/dnode of the record
+    const float *gp;  // the record's position adjoints (adjoint_gpos_kernel)
     const LineShape<R> *p;
     const ShapeCtl *ctl;
     double u;         // nu* - origin
@@ -60,6 +86,54 @@ struct LineAdj {
             acc((double)q * h, gs);
         }
     }
+    // A wing line dmin >= 1 cm-1 outside the subinterval (lw: left of it) whose nearest node
+    // is on the chi-corrected Lorentz branch (the level-0 kernel's test, so every node is):
+    // its 21 positions at |X| = dmin + q/2 (left) or dmin + (20 - q)/2 (right), weighted by
+    // the record's position adjoints, summed over the record in precision R and added to
+    // the FP64 sums once.  Returns false (nothing done) when some node is not on that branch.
+    __device__ inline bool far(double dmin, bool lw) {
+        const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
+        if (!(ctl->shape_id == 1 || (R)dmin * p->xs >= margin)) return false;
+        const R base = lw ? (R)dmin : (R)(dmin + 10.0), step = lw ? R(0.5) : R(-0.5);
+        const bool chi = ctl->mol == 2 && ctl->chi_id != 0;
+        const R spi = p->S / R(MARFA_PI_F), gl2 = p->gl2, lorc = p->lorc;
+        R sS = R(0), sX = R(0), sL = R(0), sT = R(0);
+#pragma unroll 3
+        for (int q = 0; q <= 20; q++) {
+            const R ax = q == 0 ? base : fma(step, (R)q, base);
+            const R gs = (R)gp[q];
+            const R inv = recip<R>(ax * ax + gl2);
+            R c = R(1), dc = R(0);
+            if (chi) {
+                if constexpr (sizeof(R) == 8) {
+                    c = chi_factor<double>(ctl->chi_id, ax, ctl->B1d, ctl->B2d);
+                    dc = dchi_factor<double>(ctl->chi_id, ax, ctl->B1d, ctl->B2d, c);
+                } else {
+                    c = chi_factor<float>(ctl->chi_id, ax, ctl->B1, ctl->B2);
+                    dc = dchi_factor<float>(ctl->chi_id, ax, ctl->B1, ctl->B2, c);
+                }
+            }
+            const R gl = gs * lorc * inv;  // gs x Lorentz
+            const R gv = gl * c;           // gs x sample
+            sS += gv;
+            // d/d|X| of Lorentz x chi: -2|X| Lorentz chi / (X^2 + gL^2) + Lorentz dchi/d|X|
+            sX += gl * (dc - R(2) * ax * inv * c);
+            // d/dgL: S/pi (X^2 - gL^2) / (X^2 + gL^2)^2 chi
+            sL += gs * spi * (ax * ax - gl2) * inv * inv * c;
+            if (chi && ctl->chi_id == 3 && ax > R(3)) {
+                // Perrin: d chi / dT = chi de/dT (chilor_grad)
+                const R dB1 = (R)ctl->dB1, dB2 = (R)ctl->dB2;
+                const R de = ax < R(30) ? -dB1 * (ax - R(3)) : ax < R(120) ? -R(27) * dB1 - dB2 * (ax - R(30))
+                                                                         : -R(27) * dB1 - R(90) * dB2;
+                sT += gv * de;
+            }
+        }
+        aS += (double)sS;
+        aX += lw ? -(double)sX : (double)sX;  // X = -|X| on the left wing
+        aL += (double)sL;
+        aT += (double)sT;
+        return true;
+    }
     // centerLBL (src/LineGridCalc.f90:181-494): RK(1), the two walks and the core points
     __device__ inline void centre() {
         acc(0.0, gr[node_rk(1)]);
@@ -85,6 +159,7 @@ struct LineAdj {
     // leftLBL (:7-179): the line's right wing from the subinterval's left edge
     __device__ inline void left() {
         const double d = -u;
+        if (d >= 1.0 && far(d, true)) return;
         acc(0.0, gr[node_rk(1)]);
         if (d >= 1.0) {
             walk(0, 0, 20);
@@ -109,6 +184,7 @@ struct LineAdj {
     // rightLBL (:496-674): the line's left wing from the subinterval's right edge
     __device__ inline void right() {
         const double dr = u - 10.0;
+        if (dr >= 1.0 && far(dr, false)) return;
         acc(0.0, gr[node_rk(1)]);
         if (dr >= 1.0) {
             walk(0, 0, 20);
@@ -163,6 +239,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
             const double org = a.org[r];
             w.u = nus - org;
             w.gr = b.gbar + ((int64_t)lev * a.nrec + r) * (30690 + NT);
+            w.gp = b.gpos + ((int64_t)lev * a.nrec + r) * GPOS;
             if (nus < org) {
                 if (!(-w.u > cut)) w.left();
             } else if (nus > org + 10.0) {
@@ -232,15 +309,24 @@ __global__ __launch_bounds__(BLOCK) void adjoint_reduce_kernel(const double *gli
 }
 
 // per-level dL/dT and dL/dP of a level batch: the block partials of adjoint_line_kernel
-// summed in block order; out[l] (T) and out[nlev + l] (P) for the batch's levels from l0
+// summed in a fixed order (strided per thread, then wave butterflies, then waves in order);
+// one workgroup per (level, quantity); out[l] (T) and out[nlev + l] (P) for the batch's
+// levels from l0
 __global__ __launch_bounds__(BLOCK) void adjoint_level_kernel(const double *glev, int nb, int nblk, int nlev, int l0,
                                                              double *out) {
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= 2 * nb) return;
-    const int q = t >> 1, c = t & 1;
+    const int q = blockIdx.x >> 1, c = blockIdx.x & 1;
+    if (q >= nb) return;
     double x = 0.0;
-    for (int b2 = 0; b2 < nblk; b2++) x += glev[((int64_t)q * nblk + b2) * 2 + c];
-    out[(int64_t)c * nlev + l0 + q] = x;
+    for (int b2 = threadIdx.x; b2 < nblk; b2 += BLOCK) x += glev[((int64_t)q * nblk + b2) * 2 + c];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    __shared__ double red[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0];
+        for (int w = 1; w < BLOCK / 64; w++) s += red[w];
+        out[(int64_t)c * nlev + l0 + q] = s;
+    }
 }
 
 // per-line gradients [nl][4] (lines lmin .. lmin + nl - 1) into the caller's layout

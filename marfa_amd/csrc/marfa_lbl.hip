@@ -575,6 +575,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
 #include "lbl_fine.h"
 #include "lbl_adjoint.h"
 #include "lbl_sort.h"
+#include "lbl_loss.h"
 
 // Fine-kernel variants (the batch autotuner's choices): centre lines staged in LDS per tile
 // {512, 1024, 2048} x contiguous record ranges per XCD {on, off}.  Variant 1 is the default.
@@ -717,7 +718,7 @@ struct marfa_ctx {
     // per-run scratch; a batched job borrows its lane's set (s points there)
     struct Scratch {
         Buf params[4], keys[2], idx[2], coarse, shapes, buckets, hist, spart;
-        Buf gbar, gline, glev, gacc;  // adjoint
+        Buf gbar, gpos, gline, glev, gacc;  // adjoint
     };
     Scratch own;
     Scratch *s = &own;
@@ -727,6 +728,18 @@ struct marfa_ctx {
     float timings[5] = {0, 0, 0, 0, 0};
     double *dbg_grids = nullptr;
     void *fixed_scratch = nullptr;  // this device's `scratch` (marfa_pt_tables_noalloc)
+    // Checkpoint of the last forward run: the level-0 kernel's sums (grid 0 and grids 1..5,
+    // NCOARSE values) of every record of every level, kept so that the adjoint of the same
+    // job and parameters skips that kernel.  gen changes whenever lines, levels or TIPS do.
+    Buf ckpt;
+    uint64_t gen = 0;
+    struct CkptKey {
+        bool valid = false;
+        uint64_t gen = 0;
+        marfa_job job{};
+        hipStream_t stream = nullptr;
+        int nlev = 0, nrec = 0;
+    } ck;
     std::string fixed_err;
 };
 
@@ -758,7 +771,7 @@ int ensure(marfa_ctx *c, marfa_ctx::Buf &b, size_t bytes) {
 void free_scratch(marfa_ctx::Scratch &s) {
     marfa_ctx::Buf *bufs[] = {&s.params[0], &s.params[1], &s.params[2], &s.params[3], &s.keys[0], &s.keys[1],
                               &s.idx[0], &s.idx[1], &s.coarse, &s.shapes, &s.buckets, &s.hist, &s.spart,
-                              &s.gbar, &s.gline, &s.glev, &s.gacc};
+                              &s.gbar, &s.gpos, &s.gline, &s.glev, &s.gacc};
     for (auto *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;
@@ -835,9 +848,12 @@ struct AdjReq {
 // capture: the kernels only, for recording into a hipGraph -- the level and window arrays
 // must already be on the device (an earlier run of the same job on the same context) and
 // every buffer already large enough, so that no allocation, copy or synchronisation occurs.
+// ckpt: device array [nlev][nrec][NCOARSE] for the level-0 sums instead of per-batch scratch
+// (the forward's checkpoint); reuse: it already holds them for this job, so the level-0
+// kernel is not launched.
 int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
         hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr, bool capture = false,
-        Arena *arena = nullptr) {
+        Arena *arena = nullptr, double *ckpt = nullptr, bool reuse = false) {
     if (capture) timed = false;
     const int nlev = (int)L.P.size(), nrec = W.nrec;
     if (nrec == 0 || nlev == 0) return 0;
@@ -881,7 +897,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     // level batches so that per-(line, level) arrays stay within a memory budget -- or, with
     // an arena (the allocation-free entry point), within what the arena holds
     size_t per_level = (size_t)nl * (4 * 8 + 2 * 8 + 2 * 4 + shp_bytes) + (size_t)(2 * NB + 1) * 4;
-    if (adj) per_level += (size_t)nrec * (30690 + NT) * 4 + (size_t)nl * 4 * 8;
+    if (adj) per_level += (size_t)nrec * (30690 + NT + GPOS) * 4 + (size_t)nl * 4 * 8;
     const size_t budget = c->budget;
     int lb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nlev, budget / std::max<size_t>(per_level, 1)));
     const size_t lvl_bytes = (size_t)nlev * (4 * 4 + 2 * 8) + 64, win_bytes = (size_t)nrec * 24 + 64;
@@ -944,14 +960,19 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     MARFA_GET(p_key1, c->s->keys[1], arr * 8)
     MARFA_GET(p_idx0, c->s->idx[0], arr * 4)
     MARFA_GET(p_idx1, c->s->idx[1], arr * 4)
-    MARFA_GET(p_coarse, c->s->coarse, (size_t)lb * nrec * NCOARSE * 8)
+    void *p_coarse = nullptr;
+    if (!ckpt) {
+        p_coarse = get(c->s->coarse, (size_t)lb * nrec * NCOARSE * 8);
+        if (!p_coarse) return arena ? set_err(c, MARFA_E_ARG, "fixed scratch exhausted") : MARFA_E_HIP;
+    }
     MARFA_GET(p_shapes, c->s->shapes, arr * shp_bytes)
     MARFA_GET(p_buckets, c->s->buckets, (size_t)lb * (NB + 1) * 4)
     MARFA_GET(p_hist, c->s->hist, (size_t)lb * NB * 4)
     MARFA_GET(p_spart, c->s->spart, (size_t)lb * nsblk * 4)
-    void *p_gbar = nullptr, *p_gline = nullptr, *p_glev = nullptr, *p_gacc = nullptr;
+    void *p_gbar = nullptr, *p_gpos = nullptr, *p_gline = nullptr, *p_glev = nullptr, *p_gacc = nullptr;
     if (adj) {
         if (!(p_gbar = get(c->s->gbar, (size_t)lb * nrec * (30690 + NT) * 4)) ||
+            !(p_gpos = get(c->s->gpos, (size_t)lb * nrec * GPOS * 4)) ||
             !(p_gline = get(c->s->gline, arr * 4 * 8)) || !(p_glev = get(c->s->glev, (size_t)lb * nblk * 2 * 8)) ||
             !(p_gacc = get(c->s->gacc, (size_t)nl * 4 * 8)))
             return MARFA_E_HIP;
@@ -1059,7 +1080,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         ka.shape_id = job.shape_id;
         ka.chi_id = job.chi_id;
         ka.mol = job.mol_code;
-        ka.coarse = (double *)p_coarse;
+        ka.coarse = ckpt ? ckpt + (size_t)l0 * nrec * NCOARSE : (double *)p_coarse;
         ka.out = out;
         ka.out64 = out64;
         ka.grids = c->dbg_grids;
@@ -1072,7 +1093,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         const unsigned gfine = (unsigned)(8 * ((nitems * TILES + 7) / 8));
         // the chi factor is a compile-time choice of the level-0 kernel (as in the reference,
         // src/ChiFactors.f90:11-16), so its hot loop carries only the selected branch
-        {
+        if (!reuse) {
             const bool f32 = job.precision == MARFA_FP32K;
             const int chi = job.mol_code == 2 ? job.chi_id : 0;
 #define MARFA_COARSE(CH)                                                                              \
@@ -1098,6 +1119,9 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             else
                 hipLaunchKernelGGL(adjoint_tile_kernel<double>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
             HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(adjoint_gpos_kernel, dim3((unsigned)((nitems * GPOS + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                               st, (const float *)gbar, nitems, (float *)p_gpos);
+            HIPCHK(hipGetLastError());
             AdjArgs ab;
             ab.nu = c->d_nu;
             ab.S = c->d_S;
@@ -1111,6 +1135,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             ab.P = dP;
             ab.pself = dps;
             ab.gbar = gbar;
+            ab.gpos = (const float *)p_gpos;
             ab.gline = (double *)p_gline;
             ab.glev = (double *)p_glev;
             if (job.precision == MARFA_FP32K)
@@ -1121,7 +1146,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)p_gline, nb,
                                nl, (double *)p_gacc);
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(adjoint_level_kernel, dim3((2 * nb + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st,
+            hipLaunchKernelGGL(adjoint_level_kernel, dim3(2 * nb), dim3(BLOCK), 0, st,
                                (const double *)p_glev, nb, (int)nblk, nlev, l0, adj->glevels);
             HIPCHK(hipGetLastError());
         } else if (job.precision == MARFA_FP32K) {
@@ -1190,7 +1215,7 @@ int marfa_ctx_destroy(marfa_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_scratch(c->own);
-    marfa_ctx::Buf *bufs[] = {&c->lvl, &c->win, &c->out, &c->counts, &c->adjout};
+    marfa_ctx::Buf *bufs[] = {&c->lvl, &c->win, &c->out, &c->counts, &c->adjout, &c->ckpt};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
@@ -1206,6 +1231,7 @@ int marfa_set_tips(marfa_ctx *c, int32_t niso, int32_t ntemp, const float *tips)
     if (!c->d_tips) HIPCHK(hipMalloc(&c->d_tips, 74 * 501 * 4));
     HIPCHK(hipMemcpy(c->d_tips, tips, 74 * 501 * 4, hipMemcpyHostToDevice));
     c->tips_set = true;
+    c->gen++;
     return 0;
 }
 
@@ -1249,6 +1275,7 @@ int marfa_set_lines(marfa_ctx *c, int64_t n, const double *nu, const float *S, c
     }
     c->h_nu.assign(nu, nu + n);
     c->n = n;
+    c->gen++;
     return 0;
 }
 
@@ -1258,6 +1285,7 @@ int marfa_set_levels(marfa_ctx *c, int32_t nlev, const float *P, const float *T,
     c->hP.assign(P, P + nlev);
     c->hT.assign(T, T + nlev);
     c->hrho.assign(rho, rho + nlev);
+    c->gen++;
     return 0;
 }
 
@@ -1299,6 +1327,32 @@ static LevelSet level_set(const marfa_ctx *c, const marfa_job &job) {
     return L;
 }
 
+// The checkpoint buffer for a run of job over every level of the context (null when the job
+// has no records); the key is cleared until the run that fills it has been enqueued.
+static double *ckpt_for(marfa_ctx *c, const Windows &w) {
+    c->ck.valid = false;
+    if (w.nrec == 0) return nullptr;
+    if (ensure(c, c->ckpt, c->hP.size() * (size_t)w.nrec * NCOARSE * 8)) return nullptr;
+    return (double *)c->ckpt.p;
+}
+static void ckpt_set(marfa_ctx *c, const marfa_job &job, hipStream_t st, const Windows &w) {
+    c->ck.valid = w.nrec > 0;
+    c->ck.gen = c->gen;
+    c->ck.job = job;
+    c->ck.stream = st;
+    c->ck.nlev = (int)c->hP.size();
+    c->ck.nrec = w.nrec;
+}
+// does the checkpoint hold this job's level-0 sums for the current inputs, written on the
+// same stream (so that the adjoint's kernels are ordered after the forward that wrote it)?
+static bool ckpt_holds(const marfa_ctx *c, const marfa_job &job, hipStream_t st, const Windows &w) {
+    const marfa_job &k = c->ck.job;
+    return c->ck.valid && c->ck.gen == c->gen && c->ck.stream == st && c->ck.nlev == (int)c->hP.size() &&
+           c->ck.nrec == w.nrec && k.start_wv == job.start_wv && k.end_wv == job.end_wv && k.cutoff == job.cutoff &&
+           k.shape_id == job.shape_id && k.chi_id == job.chi_id && k.mol_code == job.mol_code &&
+           k.precision == job.precision;
+}
+
 int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *out_host, void *stream,
                     int32_t async) {
     int rc = check_job(c, job);
@@ -1316,8 +1370,11 @@ int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *o
         out = (float *)c->out.p;
     }
     // phases are timed (and the host waits for them) unless the call is asynchronous
-    rc = run(c, *job, L, w, out, st, out_host != nullptr || !async);
+    double *ck = ckpt_for(c, w);
+    if (w.nrec > 0 && !ck) return MARFA_E_HIP;
+    rc = run(c, *job, L, w, out, st, out_host != nullptr || !async, nullptr, nullptr, false, nullptr, ck);
     if (rc) return rc;
+    ckpt_set(c, *job, st, w);
     if (out_host) {
         HIPCHK(hipMemcpyAsync(out_host, out, total * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1355,8 +1412,11 @@ int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, voi
     Windows w;
     compute_windows(c->h_nu, *job, w);
     LevelSet L = level_set(c, *job);
-    rc = run(c, *job, L, w, nullptr, st, true, nullptr, out_dev);
+    double *ck = ckpt_for(c, w);
+    if (w.nrec > 0 && !ck) return MARFA_E_HIP;
+    rc = run(c, *job, L, w, nullptr, st, true, nullptr, out_dev, false, nullptr, ck);
     if (rc) return rc;
+    ckpt_set(c, *job, st, w);
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -1380,8 +1440,15 @@ static int adjoint_dev(marfa_ctx *c, const marfa_job *job, const float *grad_out
     req.glines = grad_lines;
     req.glevels = grad_levels;
     // dL/dP_level and dL/dT_level: T also enters through pSelf = rho 10 k T / p0 (in the
-    // line pass) -- the density is held fixed
-    return run(c, *job, L, w, nullptr, st, timed, &req);
+    // line pass) -- the density is held fixed.  The level-0 sums come from the forward's
+    // checkpoint when it holds this job with the current inputs, else they are recomputed.
+    const bool reuse = ckpt_holds(c, *job, st, w);
+    double *ck = reuse ? (double *)c->ckpt.p : ckpt_for(c, w);
+    if (!ck) return MARFA_E_HIP;
+    const int rc = run(c, *job, L, w, nullptr, st, timed, &req, nullptr, false, nullptr, ck, reuse);
+    if (rc) return rc;
+    if (!reuse) ckpt_set(c, *job, st, w);
+    return 0;
 }
 
 int marfa_pt_tables_adjoint(marfa_ctx *c, const marfa_job *job, const float *grad_out, void *stream,
@@ -1424,6 +1491,7 @@ int marfa_update_line_params(marfa_ctx *c, const float *S, const float *gair, co
     HIPCHK(hipMemcpyAsync(c->d_gair, gair, b, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(c->d_nT, nT, b, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(c->d_delta, delta, b, hipMemcpyDeviceToDevice, st));
+    c->gen++;
     return 0;
 }
 
@@ -1596,6 +1664,52 @@ int marfa_grid_sums(marfa_ctx *c, const marfa_job *job, int32_t level, int32_t r
     if (record) HIPCHK(hipMemcpyAsync(record, c->out.p, NT * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+int log_loss(int64_t n, const T *k, const T *khat, const float *w, double wscale, double *partial_dev,
+             double *loss_dev, void *stream) {
+    if (n < 0 || (n > 0 && (!k || !khat)) || !partial_dev || !loss_dev) return MARFA_E_ARG;
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(loss_partial_kernel<T>, dim3(LOSS_BLOCKS), dim3(BLOCK), 0, st, n, k, khat, w, wscale,
+                       partial_dev);
+    hipLaunchKernelGGL(loss_final_kernel, dim3(1), dim3(BLOCK), 0, st, (const double *)partial_dev, LOSS_BLOCKS,
+                       loss_dev);
+    return hipGetLastError() == hipSuccess ? 0 : MARFA_E_HIP;
+}
+
+template <typename T>
+int log_loss_grad(int64_t n, const T *k, const T *khat, const float *w, double wscale, const double *gscale_dev,
+                  T *grad_dev, void *stream) {
+    if (n < 0 || (n > 0 && (!k || !khat || !grad_dev))) return MARFA_E_ARG;
+    if (n == 0) return 0;
+    const unsigned g = (unsigned)std::min<int64_t>((n + BLOCK - 1) / BLOCK, 8192);
+    hipLaunchKernelGGL(loss_grad_kernel<T>, dim3(g), dim3(BLOCK), 0, (hipStream_t)stream, n, k, khat, w, wscale,
+                       gscale_dev, grad_dev);
+    return hipGetLastError() == hipSuccess ? 0 : MARFA_E_HIP;
+}
+}  // namespace
+
+extern "C" {
+
+int marfa_log_loss(int64_t n, const float *k, const float *khat, const float *w, double wscale, double *partial_dev,
+                   double *loss_dev, void *stream) {
+    return log_loss<float>(n, k, khat, w, wscale, partial_dev, loss_dev, stream);
+}
+int marfa_log_loss_grad(int64_t n, const float *k, const float *khat, const float *w, double wscale,
+                        const double *gscale_dev, float *grad_dev, void *stream) {
+    return log_loss_grad<float>(n, k, khat, w, wscale, gscale_dev, grad_dev, stream);
+}
+int marfa_log_loss_f64(int64_t n, const double *k, const double *khat, const float *w, double wscale,
+                       double *partial_dev, double *loss_dev, void *stream) {
+    return log_loss<double>(n, k, khat, w, wscale, partial_dev, loss_dev, stream);
+}
+int marfa_log_loss_grad_f64(int64_t n, const double *k, const double *khat, const float *w, double wscale,
+                            const double *gscale_dev, double *grad_dev, void *stream) {
+    return log_loss_grad<double>(n, k, khat, w, wscale, gscale_dev, grad_dev, stream);
 }
 
 int marfa_last_timings(const marfa_ctx *c, float *ms5) {

@@ -120,6 +120,52 @@ void set_levels(int64_t h, const at::Tensor &P, const at::Tensor &T, const at::T
           "marfa_set_levels");
 }
 
+// the spectral loss sum w (ln k - ln khat)^2 (a 0-dim float64 tensor) and its gradient
+// dL/dk times gscale (float32, k's shape), on the current stream (marfa_log_loss[_grad])
+void check_loss_args(const at::Tensor &k, const at::Tensor &kh, const c10::optional<at::Tensor> &w) {
+    TORCH_CHECK(k.is_cuda() && (k.scalar_type() == at::kFloat || k.scalar_type() == at::kDouble) && k.is_contiguous(),
+                "marfa: records must be a contiguous float32 or float64 CUDA tensor");
+    TORCH_CHECK(kh.device() == k.device() && kh.scalar_type() == k.scalar_type() && kh.is_contiguous() &&
+                    kh.numel() == k.numel(),
+                "marfa: target must be a contiguous tensor of the records' size, dtype and GPU");
+    if (w) {
+        TORCH_CHECK(w->device() == k.device() && w->scalar_type() == at::kFloat && w->is_contiguous() &&
+                        w->numel() == k.numel(),
+                    "marfa: weights must be a contiguous float32 tensor of the records' size on their GPU");
+    }
+}
+
+at::Tensor log_loss(const at::Tensor &k, const at::Tensor &kh, const c10::optional<at::Tensor> &w, double wscale) {
+    check_loss_args(k, kh, w);
+    const auto opt = at::TensorOptions().dtype(at::kDouble).device(k.device());
+    at::Tensor part = at::empty({MARFA_LOSS_PARTIALS}, opt), loss = at::empty({}, opt);
+    const hipStream_t st = c10::hip::getCurrentHIPStream(k.device().index()).stream();
+    const float *wp = w ? w->data_ptr<float>() : nullptr;
+    const int rc = k.scalar_type() == at::kFloat
+                       ? marfa_log_loss(k.numel(), k.data_ptr<float>(), kh.data_ptr<float>(), wp, wscale,
+                                        part.data_ptr<double>(), loss.data_ptr<double>(), st)
+                       : marfa_log_loss_f64(k.numel(), k.data_ptr<double>(), kh.data_ptr<double>(), wp, wscale,
+                                            part.data_ptr<double>(), loss.data_ptr<double>(), st);
+    TORCH_CHECK(rc == 0, "marfa: marfa_log_loss failed with code ", rc);
+    return loss;
+}
+
+at::Tensor log_loss_grad(const at::Tensor &k, const at::Tensor &kh, const c10::optional<at::Tensor> &w, double wscale,
+                         const at::Tensor &gscale) {
+    check_loss_args(k, kh, w);
+    const at::Tensor g = gscale.to(k.device(), at::kDouble).reshape({1}).contiguous();
+    at::Tensor grad = at::empty_like(k);
+    const hipStream_t st = c10::hip::getCurrentHIPStream(k.device().index()).stream();
+    const float *wp = w ? w->data_ptr<float>() : nullptr;
+    const int rc = k.scalar_type() == at::kFloat
+                       ? marfa_log_loss_grad(k.numel(), k.data_ptr<float>(), kh.data_ptr<float>(), wp, wscale,
+                                             g.data_ptr<double>(), grad.data_ptr<float>(), st)
+                       : marfa_log_loss_grad_f64(k.numel(), k.data_ptr<double>(), kh.data_ptr<double>(), wp, wscale,
+                                                 g.data_ptr<double>(), grad.data_ptr<double>(), st);
+    TORCH_CHECK(rc == 0, "marfa: marfa_log_loss_grad failed with code ", rc);
+    return grad;
+}
+
 }  // namespace
 
 TORCH_LIBRARY(marfa, m) {
@@ -131,4 +177,6 @@ TORCH_LIBRARY(marfa, m) {
           &pt_tables_adjoint);
     m.def("set_line_params(int ctx, Tensor S, Tensor gair, Tensor nT, Tensor delta) -> ()", &set_line_params);
     m.def("set_levels(int ctx, Tensor P, Tensor T, Tensor rho) -> ()", &set_levels);
+    m.def("log_loss(Tensor k, Tensor khat, Tensor? w, float wscale) -> Tensor", &log_loss);
+    m.def("log_loss_grad(Tensor k, Tensor khat, Tensor? w, float wscale, Tensor gscale) -> Tensor", &log_loss_grad);
 }

@@ -105,7 +105,8 @@ EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_v
            "marfa_grid_sums", "marfa_last_timings", "marfa_batch_create", "marfa_batch_destroy",
            "marfa_batch_last_error", "marfa_batch_set_tips", "marfa_batch_add_job", "marfa_batch_tune",
            "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records",
-           "marfa_species_code", "marfa_par_convert")
+           "marfa_species_code", "marfa_par_convert", "marfa_log_loss", "marfa_log_loss_grad",
+           "marfa_log_loss_f64", "marfa_log_loss_grad_f64")
 
 
 def _p(a):
