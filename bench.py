@@ -258,8 +258,9 @@ def adjoint_bench(steps=3):
             "fwd_ms_per_level": f / nlev, "adjoint_ms_per_level": a / nlev, "levels": nlev,
             "fwd_kernels_ms": float(np.mean(kf)), "adjoint_kernels_ms": float(np.mean(ka)),
             "note": "fwd_ms / adjoint_ms: stream time of PTTables.apply and of loss.backward (parameter "
-                    "upload, loss gradient in torch, recomputation of the grid sums and the adjoint "
-                    "kernels); *_kernels_ms: HIP-event time of the library's kernels alone"}
+                    "upload, the fused loss gradient, recomputation of the grids 6-9 sums and the adjoint "
+                    "kernels; the level-0 sums come from the forward's checkpoint); *_kernels_ms: HIP-event "
+                    "time of the library's kernels alone"}
 
 
 def batch_bench(steps=2, lanes=4, check_jobs=24):
@@ -316,8 +317,10 @@ def batch_bench(steps=2, lanes=4, check_jobs=24):
 
 def large_bench(levels_checked=(0, 49, 99)):
     """8,000,000 H2O-like lines over 10-20,000 cm-1, 100 levels (P 100 -> 1e-5 atm, T 750 ->
-    160 K), cutoff 250, FP32K: forward and adjoint (grid sums recomputed per record tile, never
-    stored) of sum (log k - log k_hat)^2, k_hat from S and gamma_air perturbed by U(-5%, 5%);
+    160 K), cutoff 250, FP32K: forward and adjoint of sum (log k - log k_hat)^2, k_hat from S and
+    gamma_air perturbed by U(-5%, 5%) -- recompute-based checkpointing per subinterval: the
+    forward keeps each record's level-0 sums (15 KB per record), the adjoint recomputes the
+    finer grids per record tile, so no grid array is stored;
     points/s, kernel roofline fractions, and the error against the FP64 oracle on the
     10000-10100 cm-1 window at three levels."""
     import torch

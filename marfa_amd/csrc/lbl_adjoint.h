@@ -86,53 +86,63 @@ struct LineAdj {
             acc((double)q * h, gs);
         }
     }
-    // A wing line dmin >= 1 cm-1 outside the subinterval (lw: left of it) whose nearest node
-    // is on the chi-corrected Lorentz branch (the level-0 kernel's test, so every node is):
-    // its 21 positions at |X| = dmin + q/2 (left) or dmin + (20 - q)/2 (right), weighted by
-    // the record's position adjoints, summed over the record in precision R and added to
-    // the FP64 sums once.  Returns false (nothing done) when some node is not on that branch.
-    __device__ inline bool far(double dmin, bool lw) {
+    // Does a wing line dmin >= 1 cm-1 outside the subinterval have its nearest node -- and
+    // so every node -- on the chi-corrected Lorentz branch (the level-0 kernel's test)?
+    __device__ inline bool far_branch(double dmin) const {
         const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
-        if (!(ctl->shape_id == 1 || (R)dmin * p->xs >= margin)) return false;
+        return ctl->shape_id == 1 || (R)dmin * p->xs >= margin;
+    }
+    // Such a line (lw: left of the subinterval): its 21 positions at |X| = dmin + q/2 (left)
+    // or dmin + (20 - q)/2 (right), weighted by the record's position adjoints gp, summed
+    // over the record in precision R and added to the FP64 sums once.
+    __device__ inline void far(double dmin, bool lw) {
+        const float4 *g4 = reinterpret_cast<const float4 *>(gp);  // GPOS floats, 16-byte aligned
         const R base = lw ? (R)dmin : (R)(dmin + 10.0), step = lw ? R(0.5) : R(-0.5);
         const bool chi = ctl->mol == 2 && ctl->chi_id != 0;
         const R spi = p->S / R(MARFA_PI_F), gl2 = p->gl2, lorc = p->lorc;
         R sS = R(0), sX = R(0), sL = R(0), sT = R(0);
-#pragma unroll 3
-        for (int q = 0; q <= 20; q++) {
-            const R ax = q == 0 ? base : fma(step, (R)q, base);
-            const R gs = (R)gp[q];
-            const R inv = recip<R>(ax * ax + gl2);
-            R c = R(1), dc = R(0);
-            if (chi) {
-                if constexpr (sizeof(R) == 8) {
-                    c = chi_factor<double>(ctl->chi_id, ax, ctl->B1d, ctl->B2d);
-                    dc = dchi_factor<double>(ctl->chi_id, ax, ctl->B1d, ctl->B2d, c);
-                } else {
-                    c = chi_factor<float>(ctl->chi_id, ax, ctl->B1, ctl->B2);
-                    dc = dchi_factor<float>(ctl->chi_id, ax, ctl->B1, ctl->B2, c);
+        // positions four at a time (one 16-byte load of their adjoints)
+#pragma unroll 1
+        for (int q0 = 0; q0 <= 20; q0 += 4) {
+            const float4 x = g4[q0 >> 2];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int q = q0 + e;
+                if (q > 20) break;
+                const R gs = (R)(e == 0 ? x.x : e == 1 ? x.y : e == 2 ? x.z : x.w);
+                const R ax = fma(step, (R)q, base);
+                const R inv = recip<R>(ax * ax + gl2);
+                R c = R(1), dc = R(0);
+                if (chi) {
+                    if constexpr (sizeof(R) == 8) {
+                        c = chi_factor<double>(ctl->chi_id, ax, ctl->B1d, ctl->B2d);
+                        dc = dchi_factor<double>(ctl->chi_id, ax, ctl->B1d, ctl->B2d, c);
+                    } else {
+                        c = chi_factor<float>(ctl->chi_id, ax, ctl->B1, ctl->B2);
+                        dc = dchi_factor<float>(ctl->chi_id, ax, ctl->B1, ctl->B2, c);
+                    }
                 }
-            }
-            const R gl = gs * lorc * inv;  // gs x Lorentz
-            const R gv = gl * c;           // gs x sample
-            sS += gv;
-            // d/d|X| of Lorentz x chi: -2|X| Lorentz chi / (X^2 + gL^2) + Lorentz dchi/d|X|
-            sX += gl * (dc - R(2) * ax * inv * c);
-            // d/dgL: S/pi (X^2 - gL^2) / (X^2 + gL^2)^2 chi
-            sL += gs * spi * (ax * ax - gl2) * inv * inv * c;
-            if (chi && ctl->chi_id == 3 && ax > R(3)) {
-                // Perrin: d chi / dT = chi de/dT (chilor_grad)
-                const R dB1 = (R)ctl->dB1, dB2 = (R)ctl->dB2;
-                const R de = ax < R(30) ? -dB1 * (ax - R(3)) : ax < R(120) ? -R(27) * dB1 - dB2 * (ax - R(30))
-                                                                         : -R(27) * dB1 - R(90) * dB2;
-                sT += gv * de;
+                const R gl = gs * lorc * inv;  // gs x Lorentz
+                const R gv = gl * c;           // gs x sample
+                sS += gv;
+                // d/d|X| of Lorentz x chi: -2|X| Lorentz chi / (X^2 + gL^2) + Lorentz dchi/d|X|
+                sX += gl * (dc - R(2) * ax * inv * c);
+                // d/dgL: S/pi (X^2 - gL^2) / (X^2 + gL^2)^2 chi
+                sL += gs * spi * (ax * ax - gl2) * inv * inv * c;
+                if (chi && ctl->chi_id == 3 && ax > R(3)) {
+                    // Perrin: d chi / dT = chi de/dT (chilor_grad)
+                    const R dB1 = (R)ctl->dB1, dB2 = (R)ctl->dB2;
+                    const R de = ax < R(30) ? -dB1 * (ax - R(3))
+                                 : ax < R(120) ? -R(27) * dB1 - dB2 * (ax - R(30))
+                                               : -R(27) * dB1 - R(90) * dB2;
+                    sT += gv * de;
+                }
             }
         }
         aS += (double)sS;
         aX += lw ? -(double)sX : (double)sX;  // X = -|X| on the left wing
         aL += (double)sL;
         aT += (double)sT;
-        return true;
     }
     // centerLBL (src/LineGridCalc.f90:181-494): RK(1), the two walks and the core points
     __device__ inline void centre() {
@@ -156,10 +166,10 @@ struct LineAdj {
         }
         for (int i = fu + 2; i <= NT - 4 * er; i++) acc((double)(i - 1) * HF, gr[node_rk(i)]);
     }
-    // leftLBL (:7-179): the line's right wing from the subinterval's left edge
+    // leftLBL (:7-179): the line's right wing from the subinterval's left edge (far wings
+    // on the Lorentz branch take far() instead)
     __device__ inline void left() {
         const double d = -u;
-        if (d >= 1.0 && far(d, true)) return;
         acc(0.0, gr[node_rk(1)]);
         if (d >= 1.0) {
             walk(0, 0, 20);
@@ -181,10 +191,10 @@ struct LineAdj {
         }
         walk(0, 2, 20);
     }
-    // rightLBL (:496-674): the line's left wing from the subinterval's right edge
+    // rightLBL (:496-674): the line's left wing from the subinterval's right edge (far wings
+    // on the Lorentz branch take far() instead)
     __device__ inline void right() {
         const double dr = u - 10.0;
-        if (dr >= 1.0 && far(dr, false)) return;
         acc(0.0, gr[node_rk(1)]);
         if (dr >= 1.0) {
             walk(0, 0, 20);
@@ -210,8 +220,15 @@ struct LineAdj {
     }
 };
 
-template <typename R>
-__global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b) {
+// Two passes over the (line, level) pairs, one thread each, so that the far wings -- the
+// bulk of the samples, in few registers -- run at a high occupancy apart from the
+// register-hungry rest.  PASS 0: far wings on the Lorentz branch (far()); it writes the
+// line's gradient slots and its own per-block level partials.  PASS 1: every other deposit
+// (centre lines, wings within 1 cm-1, far wings off that branch); launched after pass 0, its
+// thread adds to the slots the same line's pass-0 thread wrote and writes its own level
+// partials.  The chain rule is linear in the sums, so each pass applies it to its own.
+template <typename R, int PASS>
+__global__ __launch_bounds__(BLOCK, PASS == 0 ? 4 : sizeof(R) == 4 ? 3 : 2) void adjoint_line_kernel(KArgs a, AdjArgs b) {
     const int lev = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     double cT = 0.0, cP = 0.0;
@@ -237,14 +254,22 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
         for (; r < a.nrec && a.lo[r] <= i; r++) {
             if (i >= a.hi[r]) continue;
             const double org = a.org[r];
+            const int64_t rr = (int64_t)lev * a.nrec + r;
             w.u = nus - org;
-            w.gr = b.gbar + ((int64_t)lev * a.nrec + r) * (30690 + NT);
-            w.gp = b.gpos + ((int64_t)lev * a.nrec + r) * GPOS;
-            if (nus < org) {
-                if (!(-w.u > cut)) w.left();
-            } else if (nus > org + 10.0) {
-                if (w.u - 10.0 < cut) w.right();
-            } else if (w.u < 10.0) {
+            w.gr = b.gbar + rr * (30690 + NT);
+            w.gp = b.gpos + rr * GPOS;
+            if (nus < org || nus > org + 10.0) {
+                const bool lw = nus < org;
+                const double d = lw ? -w.u : w.u - 10.0;
+                // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
+                if (lw ? d > cut : d >= cut) continue;
+                const bool fw = d >= 1.0 && w.far_branch(d);
+                if (PASS == 0 && fw) w.far(d, lw);
+                if (PASS == 1 && !fw) {
+                    if (lw) w.left();
+                    else w.right();
+                }
+            } else if (PASS == 1 && w.u < 10.0) {
                 w.centre();
             }
         }
@@ -255,10 +280,21 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
         const double tr = (double)MARFA_TREF_F / T, fT = pow(tr, n);
         const double dnu = w.aX + w.aD * gD / nus;     // gD is proportional to nu*
         double *gl = b.gline + o * 4;
-        gl[0] = S0 != 0.0 ? w.aS / S0 : 0.0;           // S(T) is proportional to S
-        gl[1] = w.aL * fT * (P - ps);
-        gl[2] = w.aL * gL * log(tr);
-        gl[3] = dnu * P;                               // nu* = nu + delta P
+        const double g0 = S0 != 0.0 ? w.aS / S0 : 0.0;  // S(T) is proportional to S
+        const double g1 = w.aL * fT * (P - ps);
+        const double g2 = w.aL * gL * log(tr);
+        const double g3 = dnu * P;                       // nu* = nu + delta P
+        if (PASS == 0) {
+            gl[0] = g0;
+            gl[1] = g1;
+            gl[2] = g2;
+            gl[3] = g3;
+        } else {
+            gl[0] += g0;
+            gl[1] += g1;
+            gl[2] += g2;
+            gl[3] += g3;
+        }
         // d ln S(T) / dT: partition sum (linear in the TIPS table), Boltzmann factor and
         // stimulated emission (unshifted nu)
         const int iso = b.molIso[i] / 100;
@@ -272,7 +308,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
         cT = w.aS * dlnS + w.aD * gD / (2.0 * T) + w.aL * (-n * gL / T + fT * (gs - ga) * ps / T) + w.aT;
         cP = dnu * dl + w.aL * fT * ga;
     }
-    // per-block partial sums of the level gradients, fixed order
+    // per-block partial sums of the level gradients, fixed order; glev is [pass][level][block]
     for (int off = 32; off > 0; off >>= 1) {
         cT += __shfl_xor(cT, off);
         cP += __shfl_xor(cP, off);
@@ -286,7 +322,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_line_kernel(KArgs a, AdjArgs b)
     if (threadIdx.x < 2) {
         double x = 0.0;
         for (int w2 = 0; w2 < BLOCK / 64; w2++) x += red[threadIdx.x][w2];
-        b.glev[((int64_t)lev * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = x;
+        b.glev[(((int64_t)PASS * gridDim.y + lev) * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = x;
     }
 }
 
@@ -308,16 +344,19 @@ __global__ __launch_bounds__(BLOCK) void adjoint_reduce_kernel(const double *gli
     gacc[j * 4 + 3] = s3;
 }
 
-// per-level dL/dT and dL/dP of a level batch: the block partials of adjoint_line_kernel
-// summed in a fixed order (strided per thread, then wave butterflies, then waves in order);
-// one workgroup per (level, quantity); out[l] (T) and out[nlev + l] (P) for the batch's
-// levels from l0
+// per-level dL/dT and dL/dP of a level batch: the block partials of both passes of
+// adjoint_line_kernel (glev [pass][level][block][2]) summed in a fixed order (strided per
+// thread, then wave butterflies, then waves in order); one workgroup per (level, quantity);
+// out[l] (T) and out[nlev + l] (P) for the batch's levels from l0
 __global__ __launch_bounds__(BLOCK) void adjoint_level_kernel(const double *glev, int nb, int nblk, int nlev, int l0,
                                                              double *out) {
     const int q = blockIdx.x >> 1, c = blockIdx.x & 1;
     if (q >= nb) return;
     double x = 0.0;
-    for (int b2 = threadIdx.x; b2 < nblk; b2 += BLOCK) x += glev[((int64_t)q * nblk + b2) * 2 + c];
+    for (int b2 = threadIdx.x; b2 < 2 * nblk; b2 += BLOCK) {
+        const int pass = b2 >= nblk, blk = b2 - pass * nblk;
+        x += glev[(((int64_t)pass * nb + q) * nblk + blk) * 2 + c];
+    }
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     __shared__ double red[BLOCK / 64];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;

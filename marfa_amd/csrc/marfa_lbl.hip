@@ -973,7 +973,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     if (adj) {
         if (!(p_gbar = get(c->s->gbar, (size_t)lb * nrec * (30690 + NT) * 4)) ||
             !(p_gpos = get(c->s->gpos, (size_t)lb * nrec * GPOS * 4)) ||
-            !(p_gline = get(c->s->gline, arr * 4 * 8)) || !(p_glev = get(c->s->glev, (size_t)lb * nblk * 2 * 8)) ||
+            !(p_gline = get(c->s->gline, arr * 4 * 8)) || !(p_glev = get(c->s->glev, (size_t)2 * lb * nblk * 2 * 8)) ||
             !(p_gacc = get(c->s->gacc, (size_t)nl * 4 * 8)))
             return MARFA_E_HIP;
         // cleared by a kernel, not a memset, so that the sequence can be captured
@@ -1138,10 +1138,13 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             ab.gpos = (const float *)p_gpos;
             ab.gline = (double *)p_gline;
             ab.glev = (double *)p_glev;
-            if (job.precision == MARFA_FP32K)
-                hipLaunchKernelGGL(adjoint_line_kernel<float>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
-            else
-                hipLaunchKernelGGL(adjoint_line_kernel<double>, dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+            if (job.precision == MARFA_FP32K) {
+                hipLaunchKernelGGL((adjoint_line_kernel<float, 0>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+                hipLaunchKernelGGL((adjoint_line_kernel<float, 1>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+            } else {
+                hipLaunchKernelGGL((adjoint_line_kernel<double, 0>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+                hipLaunchKernelGGL((adjoint_line_kernel<double, 1>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+            }
             HIPCHK(hipGetLastError());
             hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(nblk), dim3(BLOCK), 0, st, (const double *)p_gline, nb,
                                nl, (double *)p_gacc);
