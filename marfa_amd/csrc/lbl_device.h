@@ -478,6 +478,59 @@ __device__ inline int scan_key(bool has, int key) {
     return has ? key : (before ? k : -1);
 }
 
+// ---------------------------------------------------------------- lane scans by DPP
+// Lane moves through the VALU's data-parallel primitives (no LDS round trip, unlike a
+// shuffle): dpp_move returns in each lane the value of the lane that CTRL selects, or `old`
+// where there is none (outside the 16-lane row for row shifts) or the lane's row is not in
+// ROWS.  Controls used: row_shr:n (0x110 + n: lane - n within its row), row_bcast:15
+// (0x142: the previous row's last lane), row_bcast:31 (0x143: lane 31), wave_shl:1 (0x130:
+// lane + 1 across the wavefront).  Every lane of the wavefront must be active.
+template <int CTRL, int ROWS>
+__device__ inline int dpp_move(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ inline float dpp_move(float old, float v) {
+    return __builtin_bit_cast(float, dpp_move<CTRL, ROWS>(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWS>
+__device__ inline double dpp_move(double old, double v) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)dpp_move<CTRL, ROWS>((int)(uint32_t)o, (int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)dpp_move<CTRL, ROWS>((int)(uint32_t)(o >> 32), (int)(uint32_t)(x >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// one step of seg_scan: add the selected lane's partial sums when it has the same key
+template <int CTRL, int ROWS, typename T, int N>
+__device__ inline void seg_step(int key, T (&v)[N]) {
+    const bool take = dpp_move<CTRL, ROWS>(-2, key) == key;  // keys are >= -1: -2 never matches
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+        const T t = dpp_move<CTRL, ROWS>(T(0), v[c]);
+        v[c] = take ? v[c] + t : v[c];
+    }
+}
+
+// Key-segmented inclusive sums across the wavefront: afterwards each lane holds, per value,
+// the sum over the lanes up to and including itself that have its key.  Keys must be
+// non-decreasing along the lanes (scan_key), so each key's lanes are contiguous.  Steps:
+// shifts by 1, 2, 4, 8 within the 16-lane rows, then row 0's (row 2's) last lane into row 1
+// (row 3), then lane 31 into rows 2 and 3 -- the same partial sums in the same order on
+// every run.
+template <typename T, int N>
+__device__ inline void seg_scan(int key, T (&v)[N]) {
+    seg_step<0x111, 0xF>(key, v);
+    seg_step<0x112, 0xF>(key, v);
+    seg_step<0x114, 0xF>(key, v);
+    seg_step<0x118, 0xF>(key, v);
+    seg_step<0x142, 0xA>(key, v);
+    seg_step<0x143, 0xC>(key, v);
+}
+
+// the next lane's key (-3 in lane 63): a lane ends its key's run when this differs
+__device__ inline int next_key(int key) { return dpp_move<0x130, 0xF>(-3, key); }
+
 // ---------------------------------------------------------------- accumulators
 // Node and record sums are FP64 in both precision forms: an FP32 sample enters an FP64 sum
 // exactly, which is more accurate than a compensated FP32 pair held in the same two

@@ -355,19 +355,8 @@ __device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2]
             }
         }
         key = scan_key(has, key);
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int kp = __shfl_up(key, off);
-            R w[12];
-#pragma unroll
-            for (int q = 0; q < 12; q++) w[q] = __shfl_up(a[q], off);
-            if (lane >= off && kp == key) {
-#pragma unroll
-                for (int q = 0; q < 12; q++) a[q] += w[q];
-            }
-        }
-        const int kn = __shfl_down(key, 1);
-        const bool last = key >= 0 && (lane == 63 || kn != key);
+        seg_scan(key, a);
+        const bool last = key >= 0 && next_key(key) != key;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const int cl = key - 2 + d;  // local cell of offset d (mu - 1 + d - M0)
@@ -570,20 +559,11 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
                 }
             }
             // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)
-            R v0 = side ? a4 : a0, v1 = side ? a3 : a1, v2 = side ? a2o : a2;
-            R v3 = side ? a2 : a2o, v4 = side ? a1 : a3, v5 = side ? a0 : a4;
+            R sv[6] = {side ? a4 : a0, side ? a3 : a1, side ? a2o : a2, side ? a2 : a2o, side ? a1 : a3, side ? a0 : a4};
             key = scan_key(key >= 0, key);
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int kp = __shfl_up(key, off);
-                const R u0 = __shfl_up(v0, off), u1 = __shfl_up(v1, off), u2 = __shfl_up(v2, off);
-                const R u3 = __shfl_up(v3, off), u4 = __shfl_up(v4, off), u5 = __shfl_up(v5, off);
-                if (lane >= off && kp == key) {
-                    v0 += u0; v1 += u1; v2 += u2; v3 += u3; v4 += u4; v5 += u5;
-                }
-            }
-            const int kn = __shfl_down(key, 1);
-            const bool last = key >= 0 && (lane == 63 || kn != key);
+            seg_scan(key, sv);
+            const R v0 = sv[0], v1 = sv[1], v2 = sv[2], v3 = sv[3], v4 = sv[4], v5 = sv[5];
+            const bool last = key >= 0 && next_key(key) != key;
             const bool kside = key >= npair;  // the walk of the segment (a lane may inherit it)
             // a pair may end a segment in both walks of one chunk: left walk first, then
             // right walk, so that no two lanes update one slot in the same instruction

@@ -260,7 +260,6 @@ __device__ inline void chi_law(int b, R &coef, R &rate) {
 template <typename R, int CHI>
 __device__ inline void near_grids_lanes(bool ok, double nus, double org, const LineShape<R> &p, const ShapeCtl &ctl,
                                         double *g13, R *g45) {
-    const int lane = threadIdx.x & 63;
     const double u = nus - org;
     // classes as the deposit dispatch takes them (src/LBL.f90:81-98 on the shifted centre)
     const bool left = ok && nus < org, right = ok && nus > org + 10.0, centre = ok && !left && !right && u < 10.0;
@@ -311,20 +310,12 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
                 }
             }
             // slots P1 M1 L1 P2 M2 L2 of the pair (right walk: the mirror image)
-            R s0 = side ? v[4] : v[0], s1 = side ? v[3] : v[1], s2 = side ? v[5] : v[2];
-            R s3 = side ? v[2] : v[5], s4 = side ? v[1] : v[3], s5 = side ? v[0] : v[4];
+            R sv[6] = {side ? v[4] : v[0], side ? v[3] : v[1], side ? v[5] : v[2],
+                       side ? v[2] : v[5], side ? v[1] : v[3], side ? v[0] : v[4]};
             key = scan_key(key >= 0, key);
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int kp = __shfl_up(key, off);
-                const R t0 = __shfl_up(s0, off), t1 = __shfl_up(s1, off), t2 = __shfl_up(s2, off);
-                const R t3 = __shfl_up(s3, off), t4 = __shfl_up(s4, off), t5 = __shfl_up(s5, off);
-                if (lane >= off && kp == key) {
-                    s0 += t0; s1 += t1; s2 += t2; s3 += t3; s4 += t4; s5 += t5;
-                }
-            }
-            const int kn = __shfl_down(key, 1);
-            if (key >= 0 && (lane == 63 || kn != key)) {
+            seg_scan(key, sv);
+            const R s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4], s5 = sv[5];
+            if (key >= 0 && next_key(key) != key) {
                 const int c = 2 * key;  // 0-based cells c, c + 1 of grid k
                 if (k <= 3) {
                     g13[base + c] += (double)s0;
