@@ -191,9 +191,13 @@ int marfa_ctx_info(const marfa_ctx *ctx, int64_t *nlines, int32_t *nlev, int32_t
  * the jobs to `lanes` parallel branches (largest first onto the least loaded lane), times
  * the fine-kernel variants (centre lines staged in LDS per tile: 512/1024/2048, XCD-aware
  * record order on/off) on each job's first level and keeps the fastest, then runs every
- * job once; marfa_batch_capture records all jobs into one hipGraph (fork/join over the
- * lanes' streams); marfa_batch_launch replays it with one hipGraphLaunch.  Jobs of a lane
- * share that lane's scratch (lane_budget_gb of per-level arrays, 0 = 8 GB). */
+ * job once (eagerly, lane_budget_gb of per-level arrays per lane, 0 = 8 GB).
+ * marfa_batch_capture records all jobs into one hipGraph: the jobs' levels are cut into
+ * pieces and packed into waves; a wave prepares its pieces on the lanes (spectroscopy and
+ * sort, beside the previous wave's deposits), then runs the level-0 and the fine kernel
+ * each once per kernel variant over a device work list of every record (record tile) of
+ * every piece (one fine launch per stage capacity the jobs' tuning chose).
+ * marfa_batch_launch replays the graph with one hipGraphLaunch. */
 typedef struct marfa_batch marfa_batch;
 int marfa_batch_create(int device, int32_t lanes, double lane_budget_gb, marfa_batch **out);
 int marfa_batch_destroy(marfa_batch *b);

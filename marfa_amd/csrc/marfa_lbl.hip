@@ -346,9 +346,10 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
 // nodes (Tonkov / Pollack), chi(|X| +- 1/2) = chi(|X|) exp(-+rate/2): chi is evaluated at
 // the first node and carried to the others by that FP64 ratio, one exp per line instead of
 // one per sample.  The lines closer than 1 cm-1 take the masked per-node path with
-// X = u - q/2.  The FP32 form runs at 3 waves per SIMD, the FP64 form at 2.
+// X = u - q/2.  The FP32 form runs at 3 waves per SIMD, the FP64 form at 2.  r: the record
+// (level-batch index x records + subinterval).
 template <typename R, int CHI>
-__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(KArgs a) {
+__device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     constexpr bool kBands = CHI == 1 || CHI == 2;
     __shared__ double rho_tab[8];  // exp(-rate_b * step), step = +1/2 (even entries), -1/2
     if (kBands && threadIdx.x < 8) {
@@ -357,9 +358,6 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
         rho_tab[threadIdx.x] = exp(-(double)rate * ((threadIdx.x & 1) ? -0.5 : 0.5));
     }
     __syncthreads();
-    const int nitems = a.nlev * a.nrec;
-    const int r = xcd_item(nitems);
-    if (r >= nitems) return;
     const int lev = r / a.nrec, s = r % a.nrec;
     const double org = a.org[s];
     const int64_t lo = a.lo[s], hi = a.hi[s];
@@ -562,11 +560,21 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(K
     }
 }
 
+// One job's level batch: one workgroup per record, each XCD a contiguous range of records.
+template <typename R, int CHI>
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(KArgs a) {
+    const int nitems = a.nlev * a.nrec;
+    const int r = xcd_item(nitems);
+    if (r >= nitems) return;
+    coarse_item<R, CHI>(a, r);
+}
+
 // ------------------------------------------------------------------ 4. fine + cascade
 #include "lbl_fine.h"
 #include "lbl_adjoint.h"
 #include "lbl_sort.h"
 #include "lbl_loss.h"
+#include "lbl_multi.h"
 
 // Fine-kernel variants (the batch autotuner's choices): centre lines staged in LDS per tile
 // {512, 1024, 2048} x contiguous record ranges per XCD {on, off}.  Variant 1 is the default.
@@ -827,6 +835,179 @@ struct Arena {
     }
 };
 
+// Level geometry of a run (the same for every level batch): the line range read, the
+// bucket index's origin and size, scan chunks, records per level, line blocks.
+struct RunGeom {
+    int64_t lmin = 0, nl = 0;
+    double bbase = 0.0;
+    int NB = 0, nsblk = 0, nrec = 0;
+    unsigned nblk = 0;
+};
+// Level and window arrays of a job on the device: P, T, output multiplier, pSelf, sample
+// scale, maximum exponent of S(T) per level; window bounds and origin per record.
+struct RunArrays {
+    float *P = nullptr, *T = nullptr, *mul = nullptr;
+    double *pself = nullptr, *sscale = nullptr;
+    int *emax = nullptr;
+    int64_t *lo = nullptr, *hi = nullptr;
+    double *org = nullptr;
+};
+// Device arrays of one level batch: temporaries of the spectroscopy and the sort (par: shifted
+// centre, gD, gL, S(T) in line order; key1/idx1: scattered keys; hist, spart) and what the
+// deposit kernels read (key0/idx0: sorted keys and line indices -- first the unsorted ones --,
+// shapes: the sorted line records, buckets: the bucket index, coarse: the level-0 sums).
+struct BatchBufs {
+    void *par[4] = {}, *key0 = nullptr, *key1 = nullptr, *idx0 = nullptr, *idx1 = nullptr;
+    double *coarse = nullptr;
+    void *shapes = nullptr, *buckets = nullptr, *hist = nullptr, *spart = nullptr;
+};
+
+// The run geometry of a job's windows (bucket index over every record's reach, 1/512 cm-1
+// buckets from 4 cm-1 below the first window); false when every window is empty.
+static bool geom_of(const marfa_job &job, const Windows &W, RunGeom &g) {
+    if (W.nrec == 0) return false;
+    const int64_t lmin = *std::min_element(W.lo.begin(), W.lo.end());
+    const int64_t lmax = *std::max_element(W.hi.begin(), W.hi.end());
+    if (lmax <= lmin) return false;
+    g.lmin = lmin;
+    g.nl = lmax - lmin;
+    g.bbase = W.org.front() - (double)job.cutoff - 4.0;
+    const double btop = W.org.back() + 10.0 + (double)job.cutoff + 4.0;
+    g.NB = (int)std::ceil((btop - g.bbase) * 512.0) + 1;
+    g.nsblk = (g.NB + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    g.nrec = W.nrec;
+    g.nblk = (unsigned)((g.nl + BLOCK - 1) / BLOCK);
+    return true;
+}
+
+// The level and window arrays in their device buffers (run() writes them in this layout).
+static RunArrays arrays_at(void *lvl, void *win, int nlev, int nrec) {
+    RunArrays d;
+    d.P = (float *)lvl;
+    d.T = d.P + nlev;
+    d.mul = d.T + nlev;
+    d.pself = (double *)(((uintptr_t)(d.mul + nlev) + 15) & ~(uintptr_t)15);
+    d.sscale = d.pself + nlev;
+    d.emax = (int *)(d.sscale + nlev);
+    d.lo = (int64_t *)win;
+    d.hi = d.lo + nrec;
+    d.org = (double *)(d.hi + nrec);
+    return d;
+}
+
+// Levels [l0, l0 + nb) of a job up to the deposit kernels, enqueued on st: spectroscopy (and
+// the FP32 form's per-level sample scale), then the counting sort by shifted centre into the
+// sorted keys, line indices, line records and bucket index.  ka receives the deposit
+// kernels' arguments for these levels.  timed: c->ev[1] is recorded between the two parts.
+static int enqueue_prep(marfa_ctx *c, const marfa_job &job, const RunGeom &g, const RunArrays &d, int l0, int nb,
+                        const BatchBufs &B, float *out, double *out64, hipStream_t st, bool timed, KArgs &ka,
+                        bool launch = true) {
+    const int64_t nl = g.nl, lmin = g.lmin;
+    const int NB = g.NB, nsblk = g.nsblk;
+    const double bbase = g.bbase;
+    SpecArgs sa;
+    sa.nlev = nb;
+    sa.lmin = lmin;
+    sa.nl = nl;
+    sa.nu = c->d_nu;
+    sa.S = c->d_S;
+    sa.gair = c->d_gair;
+    sa.gself = c->d_gself;
+    sa.elow = c->d_elow;
+    sa.nT = c->d_nT;
+    sa.delta = c->d_delta;
+    sa.molIso = c->d_molIso;
+    sa.P = d.P + l0;
+    sa.T = d.T + l0;
+    sa.pself = d.pself + l0;
+    sa.tips = c->d_tips;
+    sa.nus = (double *)B.par[0];
+    sa.gD = (double *)B.par[1];
+    sa.gL = (double *)B.par[2];
+    sa.ST = (double *)B.par[3];
+    sa.keys = (uint64_t *)B.key0;
+    sa.idx = (int32_t *)B.idx0;
+    sa.emax = job.precision == MARFA_FP32K ? d.emax + l0 : nullptr;
+    if (launch) {
+    // FP32 form: per-level power-of-two sample scale from the maximum exponent of S(T);
+    // FP64 form: 1 (the exponent stays at its initial -2000)
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(BLOCK), 0, st, d.emax + l0, (int64_t)nb, -2000);
+    dim3 gs((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
+    hipLaunchKernelGGL(spectroscopy_kernel, gs, dim3(BLOCK), 0, st, sa);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(scale_kernel, dim3((nb + 63) / 64), dim3(64), 0, st, (const int *)d.emax + l0, nb,
+                       d.sscale + l0);
+    HIPCHK(hipGetLastError());
+    if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
+    {
+        // order by shifted centre: bucket counts, their scan (the bucket index), scatter,
+        // in-bucket order
+        int32_t *hist = (int32_t *)B.hist, *bst = (int32_t *)B.buckets;
+        int32_t *part = (int32_t *)B.spart;
+        const uint64_t *k0 = (const uint64_t *)B.key0;
+        const int32_t *i0 = (const int32_t *)B.idx0;
+        const int64_t nh = (int64_t)nb * NB;
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((nh + BLOCK - 1) / BLOCK, 65536)),
+                           dim3(BLOCK), 0, st, hist, nh, 0);
+        const dim3 gl((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
+        hipLaunchKernelGGL(bucket_hist_kernel, gl, dim3(BLOCK), 0, st, k0, nl, bbase, NB, hist);
+        hipLaunchKernelGGL(scan_partial_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB, nsblk,
+                           part);
+        hipLaunchKernelGGL(scan_top_kernel, dim3(1, nb), dim3(BLOCK), 0, st, part, nsblk);
+        hipLaunchKernelGGL(scan_final_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB, nsblk,
+                           (const int32_t *)part, bst);
+        hipLaunchKernelGGL(bucket_scatter_kernel, gl, dim3(BLOCK), 0, st, k0, i0, nl, bbase, NB, (const int32_t *)bst,
+                           hist, (uint64_t *)B.key1, (int32_t *)B.idx1);
+        // final order into key0 / idx0 (the unsorted copies are not read again) and the line
+        // records
+        if (job.precision == MARFA_FP32K)
+            hipLaunchKernelGGL(order_shape_kernel<float>, gl, dim3(BLOCK), 0, st, (const uint64_t *)B.key1,
+                               (const int32_t *)B.idx1, nl, bbase, NB, (const int32_t *)bst, lmin,
+                               (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
+                               (const double *)sa.ST, (const double *)d.sscale, l0, (uint64_t *)B.key0,
+                               (int32_t *)B.idx0, (LineShape<float> *)B.shapes);
+        else
+            hipLaunchKernelGGL(order_shape_kernel<double>, gl, dim3(BLOCK), 0, st, (const uint64_t *)B.key1,
+                               (const int32_t *)B.idx1, nl, bbase, NB, (const int32_t *)bst, lmin,
+                               (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
+                               (const double *)sa.ST, (const double *)d.sscale, l0, (uint64_t *)B.key0,
+                               (int32_t *)B.idx0, (LineShape<double> *)B.shapes);
+        HIPCHK(hipGetLastError());
+    }
+    }  // launch
+    ka = KArgs();
+    ka.nlev = nb;
+    ka.nrec = g.nrec;
+    ka.lev0 = l0;
+    ka.lmin = lmin;
+    ka.nl = nl;
+    ka.nus = sa.nus;
+    ka.gD = sa.gD;
+    ka.gL = sa.gL;
+    ka.S = sa.ST;
+    ka.skey = (const uint64_t *)B.key0;
+    ka.sidx = (const int32_t *)B.idx0;
+    ka.lo = d.lo;
+    ka.hi = d.hi;
+    ka.org = d.org;
+    ka.sscale = d.sscale;
+    ka.T = d.T;
+    ka.outmul = d.mul;
+    ka.cutoff = job.cutoff;
+    ka.shape_id = job.shape_id;
+    ka.chi_id = job.chi_id;
+    ka.mol = job.mol_code;
+    ka.coarse = B.coarse;
+    ka.out = out;
+    ka.out64 = out64;
+    ka.grids = c->dbg_grids;
+    ka.shp = B.shapes;
+    ka.bst = (const int32_t *)B.buckets;
+    ka.bbase = bbase;
+    ka.NB = NB;
+    return 0;
+}
+
 // Adjoint request of run(): gout = dL/d(record point) in the records' layout (device);
 // the per-line gradients accumulate in c->s->gacc ([nl][4] from line lmin) and land in
 // glines (device, [4][n]), the per-level T, P gradients in glevels (device, [2][nlev]).
@@ -877,13 +1058,14 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     const int64_t nl = lmax - lmin;
 
     // bucket index of shifted centres: 1/512 cm-1 buckets over every record's reach
-    const double bbase = W.org.front() - (double)job.cutoff - 4.0;
-    const double btop = W.org.back() + 10.0 + (double)job.cutoff + 4.0;
-    const int NB = (int)std::ceil((btop - bbase) * 512.0) + 1;
+    RunGeom G0;
+    geom_of(job, W, G0);
+    const double bbase = G0.bbase;
+    const int NB = G0.NB;
     const size_t shp_bytes = job.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
 
-    const int nsblk = (NB + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    const unsigned nblk = (unsigned)((nl + BLOCK - 1) / BLOCK);
+    const int nsblk = G0.nsblk;
+    const unsigned nblk = G0.nblk;
 
     // level batches so that per-(line, level) arrays stay within a memory budget -- or, with
     // an arena (the allocation-free entry point), within what the arena holds
@@ -913,29 +1095,26 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     void *var = get(buf, bytes);                                                                     \
     if (!var) return arena ? set_err(c, MARFA_E_ARG, "fixed scratch exhausted") : MARFA_E_HIP;
 
-    // level arrays [nlev]: P, T, outmul (float), pself, sscale (double), max exponent (int)
-    void *p_lvl = c->lvl.p;
+    // level arrays [nlev]: P, T, outmul (float), pself, sscale (double), max exponent (int);
+    // window arrays [nrec]: lo, hi, origin
+    void *p_lvl = c->lvl.p, *p_win = c->win.p;
     if (!capture) {
         p_lvl = get(c->lvl, lvl_bytes);
         if (!p_lvl) return MARFA_E_HIP;
+        p_win = get(c->win, win_bytes);
+        if (!p_win) return MARFA_E_HIP;
     }
-    float *dP = (float *)p_lvl, *dT = dP + nlev, *dmul = dT + nlev;
-    double *dps = (double *)(((uintptr_t)(dmul + nlev) + 15) & ~(uintptr_t)15), *dsc = dps + nlev;
-    int *demax = (int *)(dsc + nlev);
+    const RunArrays D = arrays_at(p_lvl, p_win, nlev, nrec);
+    float *dP = D.P, *dT = D.T, *dmul = D.mul;
+    double *dps = D.pself, *dsc = D.sscale;
+    int64_t *dlo = D.lo, *dhi = D.hi;
+    double *dorg = D.org;
     if (!capture) {
         HIPCHK(hipMemcpyAsync(dP, L.P.data(), nlev * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dT, L.T.data(), nlev * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dmul, L.outmul.data(), nlev * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
     }
-    // windows
-    void *p_win = c->win.p;
-    if (!capture) {
-        p_win = get(c->win, win_bytes);
-        if (!p_win) return MARFA_E_HIP;
-    }
-    int64_t *dlo = (int64_t *)p_win, *dhi = dlo + nrec;
-    double *dorg = (double *)(dhi + nrec);
     if (!capture) {
         HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(dhi, W.hi.data(), nrec * 8, hipMemcpyHostToDevice, st));
@@ -975,110 +1154,31 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         adj->nl = nl;
     }
 
-    // FP32 form: per-level power-of-two sample scale from the maximum exponent of S(T);
-    // FP64 form: 1 (the exponent stays at its initial -2000)
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(BLOCK), 0, st, demax, (int64_t)nlev, -2000);
-    HIPCHK(hipGetLastError());
+    RunGeom G;
+    geom_of(job, W, G);
 
     float tms[5] = {0, 0, 0, 0, 0};
     for (int l0 = 0; l0 < nlev; l0 += lb) {
         const int nb = std::min(lb, nlev - l0);
         if (timed) HIPCHK(hipEventRecord(c->ev[0], st));
-        SpecArgs sa;
-        sa.nlev = nb;
-        sa.lmin = lmin;
-        sa.nl = nl;
-        sa.nu = c->d_nu;
-        sa.S = c->d_S;
-        sa.gair = c->d_gair;
-        sa.gself = c->d_gself;
-        sa.elow = c->d_elow;
-        sa.nT = c->d_nT;
-        sa.delta = c->d_delta;
-        sa.molIso = c->d_molIso;
-        sa.P = dP + l0;
-        sa.T = dT + l0;
-        sa.pself = dps + l0;
-        sa.tips = c->d_tips;
-        sa.nus = (double *)p_par0;
-        sa.gD = (double *)p_par1;
-        sa.gL = (double *)p_par2;
-        sa.ST = (double *)p_par3;
-        sa.keys = (uint64_t *)p_key0;
-        sa.idx = (int32_t *)p_idx0;
-        sa.emax = job.precision == MARFA_FP32K ? demax + l0 : nullptr;
-        dim3 gs((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
-        hipLaunchKernelGGL(spectroscopy_kernel, gs, dim3(BLOCK), 0, st, sa);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(scale_kernel, dim3((nb + 63) / 64), dim3(64), 0, st, (const int *)demax + l0, nb, dsc + l0);
-        HIPCHK(hipGetLastError());
-        if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
-        {
-            // order by shifted centre: bucket counts, their scan (the bucket index), scatter,
-            // in-bucket order
-            int32_t *hist = (int32_t *)p_hist, *bst = (int32_t *)p_buckets;
-            int32_t *part = (int32_t *)p_spart;
-            const uint64_t *k0 = (const uint64_t *)p_key0;
-            const int32_t *i0 = (const int32_t *)p_idx0;
-            const int64_t nh = (int64_t)nb * NB;
-            hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((nh + BLOCK - 1) / BLOCK, 65536)),
-                               dim3(BLOCK), 0, st, hist, nh, 0);
-            const dim3 gl((unsigned)((nl + BLOCK - 1) / BLOCK), (unsigned)nb);
-            hipLaunchKernelGGL(bucket_hist_kernel, gl, dim3(BLOCK), 0, st, k0, nl, bbase, NB, hist);
-            hipLaunchKernelGGL(scan_partial_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB,
-                               nsblk, part);
-            hipLaunchKernelGGL(scan_top_kernel, dim3(1, nb), dim3(BLOCK), 0, st, part, nsblk);
-            hipLaunchKernelGGL(scan_final_kernel, dim3(nsblk, nb), dim3(BLOCK), 0, st, (const int32_t *)hist, NB,
-                               nsblk, (const int32_t *)part, bst);
-            hipLaunchKernelGGL(bucket_scatter_kernel, gl, dim3(BLOCK), 0, st, k0, i0, nl, bbase, NB,
-                               (const int32_t *)bst, hist, (uint64_t *)p_key1, (int32_t *)p_idx1);
-            // final order into keys[0] / idx[0] (the unsorted copies are not read again) and
-            // the line records
-            if (job.precision == MARFA_FP32K)
-                hipLaunchKernelGGL(order_shape_kernel<float>, gl, dim3(BLOCK), 0, st, (const uint64_t *)p_key1,
-                                   (const int32_t *)p_idx1, nl, bbase, NB, (const int32_t *)bst, lmin,
-                                   (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
-                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)p_key0,
-                                   (int32_t *)p_idx0, (LineShape<float> *)p_shapes);
-            else
-                hipLaunchKernelGGL(order_shape_kernel<double>, gl, dim3(BLOCK), 0, st, (const uint64_t *)p_key1,
-                                   (const int32_t *)p_idx1, nl, bbase, NB, (const int32_t *)bst, lmin,
-                                   (const double *)sa.nus, (const double *)sa.gD, (const double *)sa.gL,
-                                   (const double *)sa.ST, (const double *)dsc, l0, (uint64_t *)p_key0,
-                                   (int32_t *)p_idx0, (LineShape<double> *)p_shapes);
-            HIPCHK(hipGetLastError());
-        }
-        if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
+        BatchBufs B;
+        B.par[0] = p_par0;
+        B.par[1] = p_par1;
+        B.par[2] = p_par2;
+        B.par[3] = p_par3;
+        B.key0 = p_key0;
+        B.key1 = p_key1;
+        B.idx0 = p_idx0;
+        B.idx1 = p_idx1;
+        B.coarse = ckpt ? ckpt + (size_t)l0 * nrec * NCOARSE : (double *)p_coarse;
+        B.shapes = p_shapes;
+        B.buckets = p_buckets;
+        B.hist = p_hist;
+        B.spart = p_spart;
         KArgs ka;
-        ka.nlev = nb;
-        ka.nrec = nrec;
-        ka.lev0 = l0;
-        ka.lmin = lmin;
-        ka.nl = nl;
-        ka.nus = sa.nus;
-        ka.gD = sa.gD;
-        ka.gL = sa.gL;
-        ka.S = sa.ST;
-        ka.skey = (const uint64_t *)p_key0;
-        ka.sidx = (const int32_t *)p_idx0;
-        ka.lo = dlo;
-        ka.hi = dhi;
-        ka.org = dorg;
-        ka.sscale = dsc;
-        ka.T = dT;
-        ka.outmul = dmul;
-        ka.cutoff = job.cutoff;
-        ka.shape_id = job.shape_id;
-        ka.chi_id = job.chi_id;
-        ka.mol = job.mol_code;
-        ka.coarse = ckpt ? ckpt + (size_t)l0 * nrec * NCOARSE : (double *)p_coarse;
-        ka.out = out;
-        ka.out64 = out64;
-        ka.grids = c->dbg_grids;
-        ka.shp = p_shapes;
-        ka.bst = (const int32_t *)p_buckets;
-        ka.bbase = bbase;
-        ka.NB = NB;
+        int rc = enqueue_prep(c, job, G, D, l0, nb, B, out, out64, st, timed, ka);
+        if (rc) return rc;
+        if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
         const int nitems = nb * nrec;
         const unsigned gcoarse = (unsigned)(8 * ((nitems + 7) / 8));
         const unsigned gfine = (unsigned)(8 * ((nitems * TILES + 7) / 8));
@@ -1741,6 +1841,34 @@ struct marfa_batch {
     hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
     std::vector<hipEvent_t> join;
     float last_ms = 0;
+    // The captured form: every job's levels cut into pieces (level batches); pieces packed
+    // into waves whose deposit inputs (sorted keys, line records, bucket index, level-0 sums)
+    // fit one device arena.  A wave prepares its pieces on the lanes (spectroscopy and sort
+    // per piece, lane temporaries), then runs one level-0 launch and one fine launch per
+    // kernel variant over the work list of all its pieces (a device table of the pieces'
+    // arguments; one workgroup per record or record tile).
+    struct Piece {
+        int job = 0, l0 = 0, nb = 0, lane = 0;
+        BatchBufs B;
+        KArgs ka;
+        double work = 0;  // lines x levels x records: the order of the work lists
+    };
+    struct Launch {       // one launch over a work list
+        bool coarse = true;
+        int prec = 0, variant = 0;  // chi (level-0) or LDS stage capacity (fine)
+        std::vector<int> pieces;
+        size_t tab_off = 0, first_off = 0;  // in `tables`
+        int items = 0, grid = 0;
+    };
+    struct Wave {
+        std::vector<int> pieces;
+        std::vector<Launch> launches;
+    };
+    std::vector<Piece> pieces;
+    std::vector<Wave> waves;
+    marfa_ctx::Buf arena, tables;
+    size_t arena_budget = (size_t)96 << 30;  // both halves, at most 70% of the free memory
+    hipEvent_t done[2] = {nullptr, nullptr};  // deposits of the last wave in each half
 };
 
 namespace {
@@ -1761,6 +1889,266 @@ void release_job_ctx(marfa_ctx *c) {
     c->stream = nullptr;
     marfa_ctx_destroy(c);
 }
+}  // namespace
+
+namespace {
+// Cut every job into pieces (level batches whose deposit inputs take at most a quarter of
+// a wave's half of the arena and whose temporaries fit a lane's budget), pack them into
+// waves, give each piece a preparation lane (least prepared lines first) and its buffers
+// (wave k in half k % 2 of the arena, so that a wave is prepared while the one before it
+// deposits), build the work lists (pieces by descending work per item) and put their tables
+// on the device.  Lanes 1.. prepare, lane 0's stream deposits (one lane: it does both).
+int plan_waves(marfa_batch *b) {
+    b->pieces.clear();
+    b->waves.clear();
+    // the lanes' arrays that only the eager runs used (the captured form keeps its deposit
+    // inputs in the arena) are released first
+    for (auto *l : b->lanes) {
+        marfa_ctx::Buf *unused[] = {&l->own.keys[0], &l->own.idx[0], &l->own.coarse, &l->own.shapes, &l->own.buckets};
+        for (auto *u : unused) {
+            if (u->p) BHIPCHK(hipFree(u->p));
+            u->p = nullptr;
+            u->bytes = 0;
+        }
+    }
+    size_t fr = 0, tot = 0;
+    size_t budget = b->arena_budget;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::min(budget, fr / 10 * 7);  // room for the rest
+    const size_t half = budget / 2, quarter = half / 4;
+    const int K = (int)b->lanes.size();
+    const int p0 = K > 1 ? 1 : 0;  // first preparation lane
+    struct Need {
+        size_t keep = 0, temp = 0;  // per level: deposit inputs, temporaries
+    };
+    std::vector<RunGeom> geo(b->jobs.size());
+    std::vector<Need> need(b->jobs.size());
+    for (size_t j = 0; j < b->jobs.size(); j++) {
+        const auto &J = b->jobs[j];
+        if (!geom_of(J.spec, J.W, geo[j])) continue;
+        const RunGeom &g = geo[j];
+        const size_t shp = J.spec.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
+        need[j].keep = (size_t)g.nl * (8 + 4 + shp) + (size_t)(g.NB + 1) * 4 + (size_t)g.nrec * NCOARSE * 8 + 5 * 256;
+        need[j].temp = (size_t)g.nl * (4 * 8 + 8 + 4) + (size_t)g.NB * 4 + (size_t)g.nsblk * 4;
+        const int nlev = (int)J.L.P.size();
+        const size_t cap_keep = std::max<size_t>(1, quarter / need[j].keep);
+        const size_t cap_temp = std::max<size_t>(1, b->lane_budget / std::max<size_t>(need[j].temp, 1));
+        const int lb = (int)std::min<size_t>((size_t)nlev, std::min(cap_keep, cap_temp));
+        for (int l0 = 0; l0 < nlev; l0 += lb) {
+            marfa_batch::Piece P;
+            P.job = (int)j;
+            P.l0 = l0;
+            P.nb = std::min(lb, nlev - l0);
+            P.work = (double)g.nl * g.nrec * P.nb;
+            b->pieces.push_back(P);
+        }
+    }
+    // waves: pieces in order while their deposit inputs fit half the arena (the first wave a
+    // quarter of that, so that the deposits start early: the preparation of every later wave
+    // runs beside the deposits of the wave before it)
+    size_t used = 0, top = 0;
+    for (size_t i = 0; i < b->pieces.size(); i++) {
+        const size_t sz = need[b->pieces[i].job].keep * b->pieces[i].nb;
+        if (b->waves.empty() || used + sz > (b->waves.size() == 1 ? quarter : half)) {
+            b->waves.emplace_back();
+            used = 0;
+        }
+        b->waves.back().pieces.push_back((int)i);
+        used += sz;
+        top = std::max(top, used);
+    }
+    // device memory: the arena's two halves, and per lane the temporaries of its largest piece
+    marfa_ctx *c0 = b->lanes[0];
+    const size_t hsz = (std::max<size_t>(top, 256) + 255) & ~(size_t)255;
+    if (ensure(c0, b->arena, (b->waves.size() > 1 ? 2 : 1) * hsz)) return bset_err(b, MARFA_E_HIP, c0->err);
+    std::vector<size_t> temp_of(K, 0);
+    for (size_t w = 0; w < b->waves.size(); w++) {
+        auto &W = b->waves[w];
+        std::vector<double> load(K, 0.0);
+        std::vector<int> order = W.pieces;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return b->pieces[x].work > b->pieces[y].work; });
+        for (int i : order) {
+            const int k = p0 + (int)(std::min_element(load.begin() + p0, load.end()) - (load.begin() + p0));
+            auto &P = b->pieces[i];
+            P.lane = k;
+            load[k] += (double)geo[P.job].nl * P.nb;
+        }
+        Arena ar;
+        ar.base = (char *)b->arena.p + (w % 2) * hsz;
+        ar.size = hsz;
+        for (int i : W.pieces) {
+            auto &P = b->pieces[i];
+            const auto &J = b->jobs[P.job];
+            const RunGeom &g = geo[P.job];
+            const size_t arr = (size_t)P.nb * g.nl;
+            const size_t shp = J.spec.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
+            P.B.key0 = ar.take(arr * 8);
+            P.B.idx0 = ar.take(arr * 4);
+            P.B.shapes = ar.take(arr * shp);
+            P.B.buckets = ar.take((size_t)P.nb * (g.NB + 1) * 4);
+            P.B.coarse = (double *)ar.take((size_t)P.nb * g.nrec * NCOARSE * 8);
+            if (!P.B.key0 || !P.B.idx0 || !P.B.shapes || !P.B.buckets || !P.B.coarse)
+                return bset_err(b, MARFA_E_ARG, "a wave outgrew the batch arena");
+            temp_of[P.lane] = std::max(temp_of[P.lane], need[P.job].temp * P.nb);
+        }
+    }
+    // lane temporaries: spectroscopy outputs, scattered keys and indices, histogram, scan
+    for (int k = 0; k < K; k++) {
+        marfa_ctx *l = b->lanes[k];
+        marfa_ctx::Scratch &S = l->own;
+        size_t arr_max = 0, nb_max = 0, ns_max = 0;
+        for (const auto &P : b->pieces)
+            if (P.lane == k) {
+                arr_max = std::max(arr_max, (size_t)P.nb * geo[P.job].nl);
+                nb_max = std::max(nb_max, (size_t)P.nb * geo[P.job].NB);
+                ns_max = std::max(ns_max, (size_t)P.nb * geo[P.job].nsblk);
+            }
+        for (int q = 0; q < 4; q++)
+            if (ensure(l, S.params[q], arr_max * 8)) return bset_err(b, MARFA_E_HIP, l->err);
+        if (ensure(l, S.keys[1], arr_max * 8) || ensure(l, S.idx[1], arr_max * 4) || ensure(l, S.hist, nb_max * 4) ||
+            ensure(l, S.spart, ns_max * 4))
+            return bset_err(b, MARFA_E_HIP, l->err);
+    }
+    for (auto &P : b->pieces) {
+        marfa_ctx::Scratch &S = b->lanes[P.lane]->own;
+        for (int q = 0; q < 4; q++) P.B.par[q] = S.params[q].p;
+        P.B.key1 = S.keys[1].p;
+        P.B.idx1 = S.idx[1].p;
+        P.B.hist = S.hist.p;
+        P.B.spart = S.spart.p;
+        auto &J = b->jobs[P.job];
+        const RunArrays d = arrays_at(J.ctx->lvl.p, J.ctx->win.p, (int)J.L.P.size(), J.W.nrec);
+        enqueue_prep(J.ctx, J.spec, geo[P.job], d, P.l0, P.nb, P.B, (float *)J.ctx->out.p, nullptr, nullptr, false,
+                     P.ka, false);
+        // the deposit kernels read the sorted arrays only: the lane temporaries are not theirs
+        P.ka.nus = P.ka.gD = P.ka.gL = P.ka.S = nullptr;
+    }
+    // work lists: per wave, one level-0 launch per (form, chi) and one fine launch per (form,
+    // stage capacity); pieces by descending work per item
+    std::vector<char> host;
+    auto put = [&](const void *src, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        memcpy(host.data() + off, src, bytes);
+        return off;
+    };
+    for (auto &W : b->waves) {
+        for (int coarse = 1; coarse >= 0; coarse--) {
+            std::vector<marfa_batch::Launch> ls;
+            for (int i : W.pieces) {
+                const auto &P = b->pieces[i];
+                const auto &J = b->jobs[P.job];
+                const int prec = J.spec.precision;
+                const int var = coarse ? (J.spec.mol_code == 2 ? J.spec.chi_id : 0) : VARIANT_CAP[J.ctx->variant];
+                auto it = std::find_if(ls.begin(), ls.end(),
+                                       [&](const marfa_batch::Launch &x) { return x.prec == prec && x.variant == var; });
+                if (it == ls.end()) {
+                    ls.emplace_back();
+                    it = ls.end() - 1;
+                    it->coarse = coarse;
+                    it->prec = prec;
+                    it->variant = var;
+                }
+                it->pieces.push_back(i);
+            }
+            for (auto &L : ls) {
+                std::stable_sort(L.pieces.begin(), L.pieces.end(), [&](int x, int y) {
+                    const auto &X = b->pieces[x], &Y = b->pieces[y];
+                    return (double)geo[X.job].nl / geo[X.job].nrec > (double)geo[Y.job].nl / geo[Y.job].nrec;
+                });
+                std::vector<KArgs> tab;
+                std::vector<int> first(1, 0);
+                for (int i : L.pieces) {
+                    const auto &P = b->pieces[i];
+                    tab.push_back(P.ka);
+                    first.push_back(first.back() + P.nb * geo[P.job].nrec * (coarse ? 1 : TILES));
+                }
+                L.items = first.back();
+                L.tab_off = put(tab.data(), tab.size() * sizeof(KArgs));
+                L.first_off = put(first.data(), first.size() * sizeof(int));
+                L.grid = 8 * XCD_RUN * ((L.items + 8 * XCD_RUN - 1) / (8 * XCD_RUN));
+                W.launches.push_back(L);
+            }
+        }
+    }
+    if (ensure(c0, b->tables, std::max<size_t>(host.size(), 256))) return bset_err(b, MARFA_E_HIP, c0->err);
+    BHIPCHK(hipMemcpy(b->tables.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Record the waves into the capturing stream s0: jobs without lines (zero records), then
+// per wave the pieces' preparation forked over the lanes, joined, and the work-list
+// launches.
+int capture_waves(marfa_batch *b, hipStream_t s0) {
+    for (auto &J : b->jobs) {
+        RunGeom g;
+        if (J.W.nrec > 0 && !geom_of(J.spec, J.W, g)) {
+            const int64_t words = (int64_t)J.L.P.size() * J.W.nrec * NT;
+            hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<int64_t>((words + BLOCK - 1) / BLOCK, 65536)),
+                               dim3(BLOCK), 0, s0, (int32_t *)J.ctx->out.p, words, 0);
+        }
+    }
+    char *tb = (char *)b->tables.p;
+    // the preparation lanes join the capture; wave k's preparation waits for the deposits of
+    // wave k - 2, which read the same half of the arena (events done[k % 2])
+    if (hipEventRecord(b->fork, s0) != hipSuccess) return bset_err(b, MARFA_E_HIP, "fork");
+    for (size_t k = 1; k < b->lanes.size(); k++)
+        if (hipStreamWaitEvent(b->lanes[k]->stream, b->fork, 0) != hipSuccess) return bset_err(b, MARFA_E_HIP, "fork");
+    for (size_t w = 0; w < b->waves.size(); w++) {
+        auto &W = b->waves[w];
+        if (w >= 2)
+            for (size_t k = 1; k < b->lanes.size(); k++)
+                if (hipStreamWaitEvent(b->lanes[k]->stream, b->done[w % 2], 0) != hipSuccess)
+                    return bset_err(b, MARFA_E_HIP, "wave order");
+        for (int i : W.pieces) {
+            auto &P = b->pieces[i];
+            auto &J = b->jobs[P.job];
+            RunGeom g;
+            geom_of(J.spec, J.W, g);
+            const RunArrays d = arrays_at(J.ctx->lvl.p, J.ctx->win.p, (int)J.L.P.size(), J.W.nrec);
+            KArgs ka;
+            const int rc = enqueue_prep(J.ctx, J.spec, g, d, P.l0, P.nb, P.B, (float *)J.ctx->out.p, nullptr,
+                                        b->lanes[P.lane]->stream, false, ka);
+            if (rc) return bset_err(b, rc, J.ctx->err);
+        }
+        for (size_t k = 1; k < b->lanes.size(); k++)
+            if (hipEventRecord(b->join[k], b->lanes[k]->stream) != hipSuccess ||
+                hipStreamWaitEvent(s0, b->join[k], 0) != hipSuccess)
+                return bset_err(b, MARFA_E_HIP, "join");
+        for (const auto &L : W.launches) {
+            if (L.items == 0) continue;
+            const KArgs *tab = (const KArgs *)(tb + L.tab_off);
+            const int *first = (const int *)(tb + L.first_off);
+            const int np = (int)L.pieces.size();
+            const dim3 gr((unsigned)L.grid);
+            const bool f32 = L.prec == MARFA_FP32K;
+#define MARFA_MULTI(KERN) hipLaunchKernelGGL((KERN), gr, dim3(BLOCK), 0, s0, tab, first, np)
+            if (L.coarse) {
+                switch (L.variant) {
+                case 1: if (f32) MARFA_MULTI((coarse_multi<float, 1>)); else MARFA_MULTI((coarse_multi<double, 1>)); break;
+                case 2: if (f32) MARFA_MULTI((coarse_multi<float, 2>)); else MARFA_MULTI((coarse_multi<double, 2>)); break;
+                case 3: if (f32) MARFA_MULTI((coarse_multi<float, 3>)); else MARFA_MULTI((coarse_multi<double, 3>)); break;
+                default: if (f32) MARFA_MULTI((coarse_multi<float, 0>)); else MARFA_MULTI((coarse_multi<double, 0>)); break;
+                }
+            } else {
+                switch (L.variant) {
+                case 512: if (f32) MARFA_MULTI((fine_multi<float, 512>)); else MARFA_MULTI((fine_multi<double, 512>)); break;
+                case 2048: if (f32) MARFA_MULTI((fine_multi<float, 2048>)); else MARFA_MULTI((fine_multi<double, 2048>)); break;
+                default: if (f32) MARFA_MULTI((fine_multi<float, 1024>)); else MARFA_MULTI((fine_multi<double, 1024>)); break;
+                }
+            }
+#undef MARFA_MULTI
+            if (hipGetLastError() != hipSuccess) return bset_err(b, MARFA_E_HIP, "work-list launch");
+        }
+        if (hipEventRecord(b->done[w % 2], s0) != hipSuccess) return bset_err(b, MARFA_E_HIP, "wave order");
+    }
+    for (size_t k = 1; k < b->lanes.size(); k++)  // every lane joined before the capture ends
+        if (hipEventRecord(b->join[k], b->lanes[k]->stream) != hipSuccess ||
+            hipStreamWaitEvent(s0, b->join[k], 0) != hipSuccess)
+            return bset_err(b, MARFA_E_HIP, "join");
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1790,6 +2178,7 @@ int marfa_batch_create(int device, int32_t lanes, double lane_budget_gb, marfa_b
     }
     b->join.resize(lanes);
     for (auto &e : b->join) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    for (auto &e : b->done) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     *out = b;
     return 0;
 }
@@ -1803,9 +2192,13 @@ int marfa_batch_destroy(marfa_batch *b) {
     for (auto &j : b->jobs) release_job_ctx(j.ctx);
     for (auto *l : b->lanes) marfa_ctx_destroy(l);
     if (b->d_tips) (void)hipFree(b->d_tips);
+    if (b->arena.p) (void)hipFree(b->arena.p);
+    if (b->tables.p) (void)hipFree(b->tables.p);
     for (auto e : {b->fork, b->t0, b->t1})
         if (e) (void)hipEventDestroy(e);
     for (auto e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : b->done)
         if (e) (void)hipEventDestroy(e);
     delete b;
     return 0;
@@ -1929,31 +2322,17 @@ int marfa_batch_capture(marfa_batch *b) {
         if (!J.ready) return bset_err(b, MARFA_E_STATE, "tune the batch before capturing it");
     (void)hipSetDevice(b->device);
     for (auto *l : b->lanes) BHIPCHK(hipStreamSynchronize(l->stream));
+    const int rc = plan_waves(b);
+    if (rc) return rc;
     hipStream_t s0 = b->lanes[0]->stream;
     BHIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
-    int rc = 0;
-    if (hipEventRecord(b->fork, s0) != hipSuccess) rc = MARFA_E_HIP;
-    for (size_t k = 1; !rc && k < b->lanes.size(); k++)
-        if (hipStreamWaitEvent(b->lanes[k]->stream, b->fork, 0) != hipSuccess) rc = MARFA_E_HIP;
-    for (auto &J : b->jobs) {
-        if (rc) break;
-        marfa_ctx *lane = b->lanes[J.lane];
-        J.ctx->s = &lane->own;
-        J.ctx->stream = lane->stream;
-        rc = run(J.ctx, J.spec, J.L, J.W, (float *)J.ctx->out.p, lane->stream, false, nullptr, nullptr, true);
-        if (rc) b->err = J.ctx->err;
-    }
-    for (size_t k = 1; !rc && k < b->lanes.size(); k++) {
-        if (hipEventRecord(b->join[k], b->lanes[k]->stream) != hipSuccess ||
-            hipStreamWaitEvent(s0, b->join[k], 0) != hipSuccess)
-            rc = MARFA_E_HIP;
-    }
+    int err = capture_waves(b, s0);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s0, &g);
-    if (rc || e != hipSuccess) {
+    if (err || e != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
-        if (!rc) b->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
-        return rc ? rc : MARFA_E_HIP;
+        if (!err) b->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return err ? err : MARFA_E_HIP;
     }
     b->graph = g;
     BHIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
