@@ -23,6 +23,9 @@ RECORD_WV_SPAN = 10  # cm-1 per record
 RECORD_SIZE = POINTS_PER_RECORD * 4
 PT_EXTENSION = "ptbin"
 INFO_FILENAME = "info.txt"
+PT_DIRECTORY = "output/ptTables"        # runs, and latest_run.txt naming the last one
+PROCESSED_DIRECTORY = "output/processedData"
+PLOTS_DIRECTORY = "output/plots"
 
 
 def get_pt_name(directory: Path, level: int) -> Path:
@@ -175,3 +178,42 @@ def convert_pttable(directory: Path, level: int, out_dir: Path) -> Path:
         for vw, a in zip(vw_data, absorption_data):
             out.write(f"{vw:15.5f} {a:17.7e}\n")
     return output_file
+
+
+Y_AXIS_TITLES = {"ACS": "Absorption cross-section [cm$^{2}$ mol$^{-1}$]",
+                 "VAC": "Volume absorption coefficient [km$^{-1}$]"}
+
+
+def generate_plot(x_data, y_data, parameters, plots_dir=PLOTS_DIRECTORY) -> Path:
+    """The spectrum of plot_parser's data as an SVG figure (handlers/plot.py:13-64): log10 of
+    the values against wavenumber, titled with the molecule, level, atmosphere file and
+    cutoff, the y axis named by the target value; saved under plots_dir as the file name
+    plot_parser chose.  ValueError for a target other than ACS or VAC.  Drawn on its own
+    Figure (no pyplot state), so repeated calls do not accumulate figures."""
+    from matplotlib.backends.backend_svg import FigureCanvasSVG
+    from matplotlib.figure import Figure
+    molecule, cutoff, y_title, atm_filename, level, plot_filename = parameters
+    if y_title not in Y_AXIS_TITLES:
+        raise ValueError(f"Unknown target value {y_title}")
+    x = np.asarray(x_data, dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        y = np.log10(np.asarray(y_data, dtype=np.float64))
+    fig = Figure(figsize=(12, 6))
+    FigureCanvasSVG(fig)
+    ax = fig.add_subplot()
+    ax.plot(x, y, color="g", linestyle="-")
+    ax.set_title(f"Absorption spectrum of {molecule} at {level} level of atmosphere {atm_filename}\n"
+                 f"Cutoff is {cutoff} cm-1")
+    ax.set_xlabel("Wavenumber [cm$^{-1}$]")
+    ax.set_ylabel(Y_AXIS_TITLES[y_title])
+    ax.grid(which="major", axis="both", color="gray", alpha=0.5)
+    ax.minorticks_on()
+    ax.tick_params(axis="y", which="both", right=True)
+    out = Path(plots_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    path = out / plot_filename
+    fig.savefig(path, format="svg", bbox_inches="tight")
+    print("Success!")
+    print(f"Plot is saved as {path}")
+    return path
+

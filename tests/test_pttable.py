@@ -100,3 +100,45 @@ def test_plot_parser_and_convert(tmp_path):
     lines = out.read_text().splitlines()
     assert lines[0] == "Command-Line Arguments:" and len(lines) == 7 + 3 * NT
     assert lines[7] == f"{np.float32(4000.0):15.5f} {recs[0, 0, 0]:17.7e}"
+
+
+def test_generate_plot_writes_svg(tmp_path):
+    """handlers/plot.py:13-64: an SVG figure named as plot_parser names it, in the plots
+    directory; an unknown target value is refused before anything is written."""
+    import xml.etree.ElementTree as ET
+    recs = records(2, 3, seed=7)
+    pt.write_pt_tables(tmp_path, recs, [400, 401, 402])
+    write_info(tmp_path)
+    x, y, params = pt.plot_parser(tmp_path, 1, 4000.0, 4010.0)
+    path = pt.generate_plot(x, y, params, tmp_path / "plots")
+    assert path == tmp_path / "plots" / "CO2_4000-4010_ACS.svg"
+    root = ET.parse(path).getroot()
+    assert root.tag.endswith("svg")
+    assert any(el.tag.endswith("path") for el in root.iter())
+    with pytest.raises(ValueError):
+        pt.generate_plot(x, y, params[:2] + ("XYZ",) + params[3:5] + ("bad.svg",), tmp_path / "plots")
+    assert not (tmp_path / "plots" / "bad.svg").exists()
+
+
+def test_manage_commands(tmp_path):
+    """The reference's manage.py commands over the run latest_run.txt names."""
+    from marfa_amd import manage
+    base = tmp_path / "ptTables"
+    run = base / "CO2_test"
+    recs = records(2, 3, seed=8)
+    pt.write_pt_tables(run, recs, [400, 401, 402])
+    write_info(run)
+    (base / "latest_run.txt").write_text("CO2_test\n")
+    answers = iter(["1"])
+    out = manage.main(["convert"], ask=lambda _: next(answers), base=base, processed=tmp_path / "proc",
+                      plots=tmp_path / "plots")
+    assert out == tmp_path / "proc" / "CO2_4000-4030_1level.dat"
+    answers = iter(["2", "4010", "4030"])
+    out = manage.main(["plot"], ask=lambda _: next(answers), base=base, processed=tmp_path / "proc",
+                      plots=tmp_path / "plots")
+    assert out == tmp_path / "plots" / "CO2_4010-4030_ACS.svg" and out.exists()
+    with pytest.raises(SystemExit):
+        manage.main(["unknown"], base=base)
+    (base / "latest_run.txt").write_text("")
+    with pytest.raises(SystemExit):
+        manage.main(["convert"], ask=lambda _: "1", base=base)
