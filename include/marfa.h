@@ -191,7 +191,8 @@ int marfa_ctx_info(const marfa_ctx *ctx, int64_t *nlines, int32_t *nlev, int32_t
  * the jobs to `lanes` parallel branches (largest first onto the least loaded lane), times
  * the fine-kernel variants (centre lines staged in LDS per tile: 512/1024/2048, XCD-aware
  * record order on/off) on each job's first level and keeps the fastest, then runs every
- * job once (eagerly, lane_budget_gb of per-level arrays per lane, 0 = 8 GB).
+ * job once (eagerly, lane_budget_gb of per-level arrays per lane, 0 = 8 GB; the captured
+ * form's wave arena takes at most 12 lane budgets, 96 GB and 70 % of the free memory).
  * marfa_batch_capture records all jobs into one hipGraph: the jobs' levels are cut into
  * pieces and packed into waves; a wave prepares its pieces on the lanes (spectroscopy and
  * sort, beside the previous wave's deposits), then runs the level-0 and the fine kernel

@@ -1867,7 +1867,8 @@ struct marfa_batch {
     std::vector<Piece> pieces;
     std::vector<Wave> waves;
     marfa_ctx::Buf arena, tables;
-    size_t arena_budget = (size_t)96 << 30;  // both halves, at most 70% of the free memory
+    // both halves of the wave arena: at most 96 GB, 12 lane budgets and 70 % of the free memory
+    size_t arena_budget = (size_t)96 << 30;
     hipEvent_t done[2] = {nullptr, nullptr};  // deposits of the last wave in each half
 };
 
@@ -2159,6 +2160,7 @@ int marfa_batch_create(int device, int32_t lanes, double lane_budget_gb, marfa_b
     marfa_batch *b = new marfa_batch();
     b->device = device;
     if (lane_budget_gb > 0) b->lane_budget = (size_t)(lane_budget_gb * (double)(1u << 30));
+    b->arena_budget = std::min(b->arena_budget, 12 * b->lane_budget);
     for (int k = 0; k < lanes; k++) {
         marfa_ctx *c = nullptr;
         int rc = marfa_ctx_create(device, &c);

@@ -314,3 +314,66 @@ def test_finite_differences_config4():
         m64.close()
         m32.close()
     print(f"straddling draws replaced: {straddled}", flush=True)
+
+
+@pytest.mark.gpu
+def test_adjoint_checkpoint_reuse_and_invalidation():
+    """The adjoint reuses the forward's level-0 sums only for the inputs they were computed
+    from: reused, recomputed after the same parameters are set again, and after a forward at
+    other parameters followed by a return to these (no forward), the gradients are the same."""
+    from marfa_amd.autograd import PTTableModel
+    job = adjoint_job(n=1500, nlev=2)
+    model = PTTableModel(job, TIPS)
+    try:
+        p = model.params()
+        rec = model.records()  # forward: the checkpoint holds these parameters
+        g = torch.randn(rec.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+        a1 = model.adjoint(g)  # reuses it
+        model.set_params(*p)   # same values, new generation: recomputed
+        a2 = model.adjoint(g)
+        model.set_params(*perturbed(p, np.random.default_rng(2)))
+        model.records()        # checkpoint of other parameters
+        model.set_params(*p)   # back, without a forward
+        a3 = model.adjoint(g)
+    finally:
+        model.close()
+    for a in (a2, a3):
+        assert torch.equal(a1[0], a[0]) and torch.equal(a1[1], a[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_loss_against_numpy(dtype):
+    """marfa_log_loss / marfa_log_loss_grad against the same sums in numpy (FP64), with
+    zeros and negative values (excluded points), a weight tensor and a scalar weight;
+    repeated calls give the same bits."""
+    from marfa_amd import ops
+    rng = np.random.default_rng(3)
+    n = 300_001
+    k = 10.0 ** rng.uniform(-30, -20, n)
+    kh = k * (1 + rng.uniform(-0.2, 0.2, n))
+    k[rng.integers(0, n, 500)] = 0.0
+    kh[rng.integers(0, n, 500)] = -1.0
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    kt = torch.tensor(k, dtype=dtype, device="cuda")
+    kht = torch.tensor(kh, dtype=dtype, device="cuda")
+    wt = torch.tensor(w, device="cuda")
+    kk, hh = kt.double().cpu().numpy(), kht.double().cpu().numpy()
+    m = (kk > 0) & (hh > 0)
+    d = np.where(m, np.log(np.where(m, kk, 1.0)) - np.log(np.where(m, hh, 1.0)), 0.0)
+    want = float(np.sum(0.5 * w.astype(np.float64) * d * d))
+    got = ops.load().log_loss(kt, kht, wt, 0.5)
+    assert abs(float(got) - want) <= 1e-12 * want
+    assert torch.equal(got, ops.load().log_loss(kt, kht, wt, 0.5))
+    gs = torch.tensor([3.0], dtype=torch.float64, device="cuda")
+    grad = ops.load().log_loss_grad(kt, kht, wt, 0.5, gs).double().cpu().numpy()
+    want_g = np.where(m, 3.0 * 2.0 * 0.5 * w.astype(np.float64) * d / np.where(m, kk, 1.0), 0.0)
+    # the residual d = ln k - ln k_hat cancels: an ulp of either logarithm (|ln k| ~ 60) is an
+    # absolute error of d, hence of dL/dk relative to 3 w / k
+    tol = 1e-14 if dtype == torch.float64 else 1e-6
+    slack = np.where(m, 3.0 * w.astype(np.float64) * 4.0 * np.finfo(np.float64).eps * np.abs(np.log(np.where(m, kk, 1.0)))
+                     / np.where(m, kk, 1.0), 0.0)
+    assert np.all(np.abs(grad - want_g) <= tol * np.abs(want_g) + slack)
+    assert np.all(grad[~m] == 0.0)
+    got1 = ops.load().log_loss(kt, kht, None, 1.0)
+    assert abs(float(got1) - float(np.sum(d * d))) <= 1e-12 * float(np.sum(d * d))

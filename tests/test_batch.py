@@ -94,3 +94,38 @@ def test_batch_jobs_against_oracle():
             assert err < 1e-4
     finally:
         b.close()
+
+
+def test_batch_waves_and_pieces_match_single_jobs():
+    """A lane budget of 1/4 MiB cuts the jobs into one-level pieces and the arena (12 lane
+    budgets) into several waves, so the graph prepares waves beside the previous wave's
+    deposits and runs several work lists per kernel variant: the records still equal the
+    jobs run one by one."""
+    from marfa_amd import lbl
+    jobs = uneven_jobs()[:4]
+    ctx = lbl.LBLContext(0)
+    single = []
+    try:
+        for job in jobs:
+            ctx.load_job(job, TIPS)
+            single.append(ctx.pt_tables(lbl.JobSpec.from_job(job, lbl.FP32K)))
+    finally:
+        ctx.close()
+    b = lbl.Batch(0, lanes=3, lane_budget_gb=1.0 / 4096)
+    try:
+        b.set_tips(TIPS)
+        ids = [b.add_job(job) for job in jobs]
+        b.tune()
+        b.capture()
+        for rep in range(2):
+            for i in ids:
+                nlev, nrec = C.c_int32(), C.c_int32()
+                ptr = C.c_void_p()
+                b._check(b.lib.marfa_batch_info(b.h, i, C.byref(nlev), C.byref(nrec), C.byref(ptr), None))
+                hip_memset(ptr.value, nlev.value * nrec.value * lbl.NT * 4)
+            ms = b.launch()
+            print(f"replay {rep}: {ms:.3f} ms", flush=True)
+            for i in ids:
+                assert np.array_equal(b.records(i), single[i]), f"replay {rep}, job {i}"
+    finally:
+        b.close()
