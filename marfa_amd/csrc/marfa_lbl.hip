@@ -395,6 +395,16 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     Acc pos[21];
 #pragma unroll
     for (int q = 0; q <= 20; q++) pos[q].zero();
+    // FP32 form without chi (the host launches CHI 0 whenever chi does not apply): the far
+    // lines' Lorentz samples are first summed in FP32 over
+    // FAR_RUN lines of the thread (positive terms: relative error <= FAR_RUN ulp), then added
+    // to the FP64 sums -- an FP32 multiply-add per sample instead of a conversion and an FP64
+    // multiply-add
+    constexpr int FAR_RUN = 8;
+    R run[21];
+    int nrun = 0;
+#pragma unroll
+    for (int q = 0; q <= 20; q++) run[q] = R(0);
     const int nleft = Ja - J0, nfar = nleft + (J1 - Jb);
     auto far_j = [&](int v) { return v < nleft ? J0 + v : Jb + (v - nleft); };
     // The far path reads five fields of a line record (centre, index and the three Lorentz
@@ -438,11 +448,27 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
             // the Lorentz numerator (times chi) is carried as one FP64 factor, so a sample
             // costs the reciprocal, its conversion and one FP64 multiply-add
             if (nochi) {
-                const double lorc = (double)f.lorc;
+                if constexpr (sizeof(R) == 4 && CHI == 0) {
 #pragma unroll
-                for (int q = 0; q <= 20; q++) {
-                    const R ax = q == 0 ? base : fma(step, (R)q, base);
-                    pos[q].add((double)recip<R>(ax * ax + f.gl2) * lorc);
+                    for (int q = 0; q <= 20; q++) {
+                        const R ax = q == 0 ? base : fma(step, (R)q, base);
+                        run[q] = fma(f.lorc, recip<R>(ax * ax + f.gl2), run[q]);
+                    }
+                    if (++nrun == FAR_RUN) {
+#pragma unroll
+                        for (int q = 0; q <= 20; q++) {
+                            pos[q].add((double)run[q]);
+                            run[q] = R(0);
+                        }
+                        nrun = 0;
+                    }
+                } else {
+                    const double lorc = (double)f.lorc;
+#pragma unroll
+                    for (int q = 0; q <= 20; q++) {
+                        const R ax = q == 0 ? base : fma(step, (R)q, base);
+                        pos[q].add((double)recip<R>(ax * ax + f.gl2) * lorc);
+                    }
                 }
                 continue;
             }
@@ -477,6 +503,10 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
     }
 
+    if (sizeof(R) == 4 && CHI == 0 && nrun > 0) {
+#pragma unroll
+        for (int q = 0; q <= 20; q++) pos[q].add((double)run[q]);
+    }
     // node q: position q/2 of the subinterval; even q is L of cell q/2 and P of cell q/2+1
     Acc aR, aP[10], aM[10], aL[10];
     aR = pos[0];
