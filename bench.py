@@ -263,7 +263,7 @@ def adjoint_bench(steps=3):
                     "time of the library's kernels alone"}
 
 
-def batch_bench(steps=2, lanes=4, check_jobs=24):
+def batch_bench(steps=2, lanes=2, check_jobs=24):
     """256 independent jobs (line counts 10^U(3,6), 10-100 levels, 20-500 cm-1 windows anywhere
     in 10-20,000 cm-1, molar masses {18,44,16,28,48}, cutoffs {25,125,250}), FP32K, recorded
     into one hipGraph after the autotuner's choice per job; aggregate points/s of a replay and

@@ -337,6 +337,33 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
     }
 }
 
+// The 21 grid-0 nodes of a far wing, |X_q| = base + q step, in order: use(q, 1 / (X_q^2 + gL^2)).
+// The FP32 form takes one hardware reciprocal per pair of nodes, of the product of their two
+// denominators, and forms each reciprocal from it (1/a = b / (ab): about 2 ulp instead of 1;
+// the denominators lie in [1, 7e4], so the product cannot overflow); the FP64 form divides.
+template <typename R, typename F>
+__device__ inline void far_nodes(R base, R step, R gl2, F use) {
+#pragma unroll
+    for (int q = 0; q <= 20; q += 2) {
+        const R ax0 = q == 0 ? base : fma(step, (R)q, base);
+        const R d0 = ax0 * ax0 + gl2;
+        if (q == 20) {
+            use(q, recip<R>(d0));
+        } else {
+            const R ax1 = fma(step, (R)(q + 1), base);
+            const R d1 = ax1 * ax1 + gl2;
+            if constexpr (sizeof(R) == 4) {
+                const R r = recip<R>(d0 * d1);
+                use(q, d1 * r);
+                use(q + 1, d0 * r);
+            } else {
+                use(q, recip<R>(d0));
+                use(q + 1, recip<R>(d1));
+            }
+        }
+    }
+}
+
 // One workgroup per record.  Threads take the level's sorted line records (shifted centre,
 // line index and the precomputed profile parameters) in order, so the 64 lines of a
 // wavefront lie within a fraction of a cm-1 and take the same branches.  A line at least
@@ -449,11 +476,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
             // costs the reciprocal, its conversion and one FP64 multiply-add
             if (nochi) {
                 if constexpr (sizeof(R) == 4 && CHI == 0) {
-#pragma unroll
-                    for (int q = 0; q <= 20; q++) {
-                        const R ax = q == 0 ? base : fma(step, (R)q, base);
-                        run[q] = fma(f.lorc, recip<R>(ax * ax + f.gl2), run[q]);
-                    }
+                    far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { run[q] = fma(f.lorc, inv, run[q]); });
                     if (++nrun == FAR_RUN) {
 #pragma unroll
                         for (int q = 0; q <= 20; q++) {
@@ -464,11 +487,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
                     }
                 } else {
                     const double lorc = (double)f.lorc;
-#pragma unroll
-                    for (int q = 0; q <= 20; q++) {
-                        const R ax = q == 0 ? base : fma(step, (R)q, base);
-                        pos[q].add((double)recip<R>(ax * ax + f.gl2) * lorc);
-                    }
+                    far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { pos[q].add((double)inv * lorc); });
                 }
                 continue;
             }
@@ -479,12 +498,10 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
                     chi_law<R, CHI>(b, coef, rate);
                     double lchi = (double)f.lorc * (double)(coef * dexp<R>(-(rate * base)));
                     const double rho = rho_tab[2 * b + (lw ? 0 : 1)];
-#pragma unroll
-                    for (int q = 0; q <= 20; q++) {
-                        const R ax = q == 0 ? base : fma(step, (R)q, base);
+                    far_nodes<R>(base, step, f.gl2, [&](int q, R inv) {
                         if (q > 0) lchi *= rho;
-                        pos[q].add((double)recip<R>(ax * ax + f.gl2) * lchi);
-                    }
+                        pos[q].add((double)inv * lchi);
+                    });
                     continue;
                 }
             }
