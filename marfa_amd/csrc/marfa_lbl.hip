@@ -337,29 +337,35 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
     }
 }
 
-// The 21 grid-0 nodes of a far wing, |X_q| = base + q step, in order: use(q, 1 / (X_q^2 + gL^2)).
-// The FP32 form takes one hardware reciprocal per pair of nodes, of the product of their two
-// denominators, and forms each reciprocal from it (1/a = b / (ab): about 2 ulp instead of 1;
-// the denominators lie in [1, 7e4], so the product cannot overflow); the FP64 form divides.
+// The 21 grid-0 nodes of a far wing in the FP64 form, |X_q| = base + q step, in order:
+// use(q, 1 / (X_q^2 + gL^2)).
 template <typename R, typename F>
 __device__ inline void far_nodes(R base, R step, R gl2, F use) {
 #pragma unroll
+    for (int q = 0; q <= 20; q++) {
+        const R ax = q == 0 ? base : fma(step, (R)q, base);
+        use(q, recip<R>(ax * ax + gl2));
+    }
+}
+
+// The FP32 form's far wing of one line into per-node run sums: run[q] += F / d_q, d_q =
+// X_q^2 + gL^2, |X_q| = base + q step, with one hardware reciprocal per pair of nodes:
+// F / d_q = (F / (d_q d_q')) d_q'.  F carries a factor 2^32 (the caller's), which keeps
+// F / (d_q d_q') a normal number for every F the records hold (d_q d_q' <= ~5e9 at the
+// largest cutoffs in use, d >= 1).
+__device__ inline void far_run(float base, float step, float gl2, float F, float (&run)[21]) {
+#pragma unroll
     for (int q = 0; q <= 20; q += 2) {
-        const R ax0 = q == 0 ? base : fma(step, (R)q, base);
-        const R d0 = ax0 * ax0 + gl2;
+        const float ax0 = q == 0 ? base : fmaf(step, (float)q, base);
+        const float d0 = ax0 * ax0 + gl2;
         if (q == 20) {
-            use(q, recip<R>(d0));
+            run[q] = fmaf(F, recip<float>(d0), run[q]);
         } else {
-            const R ax1 = fma(step, (R)(q + 1), base);
-            const R d1 = ax1 * ax1 + gl2;
-            if constexpr (sizeof(R) == 4) {
-                const R r = recip<R>(d0 * d1);
-                use(q, d1 * r);
-                use(q + 1, d0 * r);
-            } else {
-                use(q, recip<R>(d0));
-                use(q + 1, recip<R>(d1));
-            }
+            const float ax1 = fmaf(step, (float)(q + 1), base);
+            const float d1 = ax1 * ax1 + gl2;
+            const float fr = F * recip<float>(d0 * d1);
+            run[q] = fmaf(fr, d1, run[q]);
+            run[q + 1] = fmaf(fr, d0, run[q + 1]);
         }
     }
 }
@@ -371,15 +377,33 @@ __device__ inline void far_nodes(R base, R step, R gl2, F use) {
 // into all 21 nodes at |X| = d + q/2 (left wing) or d + (20-q)/2 (right wing), taken in
 // precision R from d >= 1 (relative error ~1 ulp).  When chi keeps one band over those
 // nodes (Tonkov / Pollack), chi(|X| +- 1/2) = chi(|X|) exp(-+rate/2): chi is evaluated at
-// the first node and carried to the others by that FP64 ratio, one exp per line instead of
-// one per sample.  The lines closer than 1 cm-1 take the masked per-node path with
-// X = u - q/2.  The FP32 form runs at 3 waves per SIMD, the FP64 form at 2.  r: the record
-// (level-batch index x records + subinterval).
+// the first node only, and the ratio to the other nodes is applied to the node sums (FP32
+// form: per run of lines, from a table) or carried node to node (FP64 form).  The lines
+// closer than 1 cm-1 take the masked per-node path with X = u - q/2.  The FP32 form runs
+// at 3 waves per SIMD, the FP64 form at 2.  r: the record (level-batch index x records +
+// subinterval).
 template <typename R, int CHI>
 __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     constexpr bool kBands = CHI == 1 || CHI == 2;
-    __shared__ double rho_tab[8];  // exp(-rate_b * step), step = +1/2 (even entries), -1/2
-    if (kBands && threadIdx.x < 8) {
+    constexpr bool kRuns = sizeof(R) == 4;
+    // FP64 form: exp(-rate_b step), step = +1/2 (even entries), -1/2.  FP32 form: the run
+    // keys' node factors 2^-32 exp(-rate_b step q) (key 2b + 0 / 1: band b, left / right
+    // wing; the last key: no chi)
+    constexpr int NKEY = kBands ? 9 : 1;
+    __shared__ double rho_tab[8];
+    __shared__ double node_fac[NKEY][21];
+    if constexpr (kRuns) {
+        for (int t = threadIdx.x; t < NKEY * 21; t += BLOCK) {
+            const int key = t / 21, q = t % 21;
+            double f = 0x1p-32;
+            if (key < NKEY - 1) {
+                R coef, rate;
+                chi_law<R, CHI>(key >> 1, coef, rate);
+                f *= exp(-(double)rate * ((key & 1) ? -0.5 : 0.5) * q);
+            }
+            node_fac[key][q] = f;
+        }
+    } else if (kBands && threadIdx.x < 8) {
         R coef, rate;
         chi_law<R, CHI>(threadIdx.x >> 1, coef, rate);
         rho_tab[threadIdx.x] = exp(-(double)rate * ((threadIdx.x & 1) ? -0.5 : 0.5));
@@ -388,6 +412,9 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     const int lev = r / a.nrec, s = r % a.nrec;
     const double org = a.org[s];
     const int64_t lo = a.lo[s], hi = a.hi[s];
+    // line indices are int32: the window test as one unsigned compare
+    const int32_t lo32 = (int32_t)lo;
+    const uint32_t span = hi > lo ? (uint32_t)(hi - lo) : 0u;
     const double cut = (double)a.cutoff;
     const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
     const bool nochi = CHI == 0 || ctl.mol != 2;
@@ -422,16 +449,26 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     Acc pos[21];
 #pragma unroll
     for (int q = 0; q <= 20; q++) pos[q].zero();
-    // FP32 form without chi (the host launches CHI 0 whenever chi does not apply): the far
-    // lines' Lorentz samples are first summed in FP32 over
-    // FAR_RUN lines of the thread (positive terms: relative error <= FAR_RUN ulp), then added
-    // to the FP64 sums -- an FP32 multiply-add per sample instead of a conversion and an FP64
-    // multiply-add
+    // FP32 form: the far lines without chi, and those whose chi keeps one band over the
+    // subinterval, are first summed per node in FP32 over runs of at most FAR_RUN lines of
+    // the thread with one key (band and wing side): a line adds F / d_q with F = 2^32 lorc
+    // chi(|X_0|), and a run enters the FP64 node sums times the key's node factor 2^-32
+    // exp(-+rate q / 2) = chi(|X_q|) / chi(|X_0|) (positive terms: relative error about
+    // FAR_RUN + 3 ulp) -- an FP32 multiply-add per sample instead of a conversion, an FP64
+    // multiply-add and the FP64 chi carry
     constexpr int FAR_RUN = 8;
     R run[21];
-    int nrun = 0;
+    int nrun = 0, rkey = NKEY - 1;
 #pragma unroll
     for (int q = 0; q <= 20; q++) run[q] = R(0);
+    auto flush = [&]() {
+#pragma unroll
+        for (int q = 0; q <= 20; q++) {
+            pos[q].add((double)run[q] * (NKEY == 1 ? 0x1p-32 : node_fac[rkey][q]));
+            run[q] = R(0);
+        }
+        nrun = 0;
+    };
     const int nleft = Ja - J0, nfar = nleft + (J1 - Jb);
     auto far_j = [&](int v) { return v < nleft ? J0 + v : Jb + (v - nleft); };
     // The far path reads five fields of a line record (centre, index and the three Lorentz
@@ -452,6 +489,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         f.gl2 = q.gl2;
         return f;
     };
+    int vslow0 = 0x7fffffff, vslow1 = -1;  // the FP32 form's lines for the second pass
     Far n1{}, n2{};
     if ((int)threadIdx.x < nfar) n1 = far_at(threadIdx.x);
     if ((int)threadIdx.x + BLOCK < nfar) n2 = far_at(threadIdx.x + BLOCK);
@@ -460,35 +498,53 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         const Far f = n1;
         n1 = n2;
         if (v + 2 * BLOCK < nfar) n2 = far_at(v + 2 * BLOCK);
-        if (f.idx < lo || f.idx >= hi) continue;  // the record's window (unshifted centres)
+        if ((uint32_t)(f.idx - lo32) >= span) continue;  // the record's window (unshifted centres)
         const double u = f.nus - org;
-        const bool lw = f.nus < org;
+        const bool lw = v < nleft;  // the lines before Ja lie left of the subinterval, those from Jb right
         // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
         const double dmin = lw ? -u : u - 10.0;
         if (lw ? dmin > cut : dmin >= cut) continue;
+        const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
+        const R step = lw ? R(0.5) : R(-0.5);
         // every node, every role: when even the nearest node, |X| >= dmin, is on the Lorentz
         // branch (|X| xs >= margin as shape_eval tests it; rounding is monotone, so every
         // node is), the nodes are summed without per-sample branch tests
-        if (ctl.shape_id == 1 || (R)dmin * f.xs >= margin) {
-            const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
-            const R step = lw ? R(0.5) : R(-0.5);
-            // the Lorentz numerator (times chi) is carried as one FP64 factor, so a sample
-            // costs the reciprocal, its conversion and one FP64 multiply-add
-            if (nochi) {
-                if constexpr (sizeof(R) == 4 && CHI == 0) {
-                    far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { run[q] = fma(f.lorc, inv, run[q]); });
-                    if (++nrun == FAR_RUN) {
-#pragma unroll
-                        for (int q = 0; q <= 20; q++) {
-                            pos[q].add((double)run[q]);
-                            run[q] = R(0);
-                        }
-                        nrun = 0;
-                    }
-                } else {
-                    const double lorc = (double)f.lorc;
-                    far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { pos[q].add((double)inv * lorc); });
+        const bool lor = ctl.shape_id == 1 || (R)dmin * f.xs >= margin;
+        if constexpr (kRuns) {
+            // one chi law (or none) over the 21 nodes: far_run into the runs; the lines that
+            // need chi or the shape per node are left to the second pass below (the thread's
+            // range [vslow0, vslow1] of them)
+            int key = NKEY - 1;
+            R F = f.lorc * R(0x1p32f);
+            bool one_law = lor && nochi;
+            if constexpr (kBands) {
+                // Pollack: |X| >= dmin >= 1 puts the far node 10 cm-1 on in band 2, so one band
+                // means band 2 at the nearest node
+                const int b = CHI == 2 ? ((R)dmin >= R(10) ? 2 : -1) : chi_band<R, CHI>((R)dmin);
+                if (lor && !nochi && (CHI == 2 ? b == 2 : b == chi_band<R, CHI>((R)(dmin + 10.0)))) {
+                    R coef, rate;
+                    chi_law<R, CHI>(CHI == 2 ? 2 : b, coef, rate);
+                    F = f.lorc * (coef * R(0x1p32f) * dexp<R>(-(rate * base)));
+                    key = 2 * b + (lw ? 0 : 1);
+                    one_law = true;
                 }
+            }
+            if (!one_law) {
+                vslow0 = min(vslow0, v);
+                vslow1 = v;
+                continue;
+            }
+            if ((key != rkey && nrun > 0) || nrun == FAR_RUN) flush();
+            rkey = key;
+            nrun++;
+            far_run(base, step, f.gl2, F, run);
+            continue;
+        }
+        if (lor) {
+            // the FP64 form: the Lorentz numerator (times chi) is one FP64 factor
+            if (nochi) {
+                const double lorc = (double)f.lorc;
+                far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { pos[q].add((double)inv * lorc); });
                 continue;
             }
             if constexpr (kBands) {
@@ -505,7 +561,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
                     continue;
                 }
             }
-            LineShape<R> pl;  // chilor_abs reads the Lorentz fields only
+            LineShape<R> pl;
             pl.lorc = f.lorc;
             pl.gl2 = f.gl2;
 #pragma unroll
@@ -520,9 +576,39 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
     }
 
-    if (sizeof(R) == 4 && CHI == 0 && nrun > 0) {
+    if (kRuns && nrun > 0) flush();
+    if constexpr (kRuns) {
+        // second pass of the FP32 form: the far lines with chi over several bands (chi per
+        // node) or off the Lorentz branch (the shape per node), straight into the FP64 sums
+        for (int v = vslow0; v <= vslow1; v += BLOCK) {
+            const Far f = far_at(v);
+            if ((uint32_t)(f.idx - lo32) >= span) continue;
+            const double u = f.nus - org;
+            const bool lw = v < nleft;
+            const double dmin = lw ? -u : u - 10.0;
+            if (lw ? dmin > cut : dmin >= cut) continue;
+            const R base = lw ? (R)dmin : (R)(dmin + 10.0);
+            const R step = lw ? R(0.5) : R(-0.5);
+            const bool lor = ctl.shape_id == 1 || (R)dmin * f.xs >= margin;
+            if (lor) {
+                if (nochi) continue;
+                if constexpr (kBands) {
+                    if (chi_band<R, CHI>((R)dmin) == chi_band<R, CHI>((R)(dmin + 10.0))) continue;
+                }
+                LineShape<R> pl;  // chilor_abs reads the Lorentz fields only
+                pl.lorc = f.lorc;
+                pl.gl2 = f.gl2;
 #pragma unroll
-        for (int q = 0; q <= 20; q++) pos[q].add((double)run[q]);
+                for (int q = 0; q <= 20; q++) {
+                    const R ax = q == 0 ? base : fma(step, (R)q, base);
+                    pos[q].add((double)chilor_abs<R, CHI>(ax, pl, ctl));
+                }
+            } else {
+                const LineShape<R> p = shp[far_j(v)];
+#pragma unroll
+                for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
+            }
+        }
     }
     // node q: position q/2 of the subinterval; even q is L of cell q/2 and P of cell q/2+1
     Acc aR, aP[10], aM[10], aL[10];
