@@ -263,11 +263,16 @@ __constant__ static const float ASYM_U[9] = {1.f, 1.5f, 2.f, 2.5f, 3.f, 3.5f, 4.
 __constant__ static const float ASYM_W[9] = {-0.688f, 0.2667f, 0.6338f, 0.4405f, 0.2529f,
                                              0.1601f, 0.1131f, 0.0853f, 0.068f};
 
-// Lorentz x chi (src/Spectroscopy.f90:202,227)
-template <typename R, int CHI = -1>
+// Lorentz x chi (src/Spectroscopy.f90:202,227).  NEAR: the caller samples within a few
+// tenths of a cm-1 of the centre; every chi law is exactly 1 for |X| < 3
+// (src/ChiFactors.f90), so chi is evaluated only from there on.
+template <typename R, int CHI = -1, bool NEAR = false>
 __device__ inline R chilor(R X, const LineShape<R> &p, const ShapeCtl &c) {
     R lor = p.lorc * recip<R>(X * X + p.gl2);
-    if constexpr (CHI != 0) lor *= chi_sel<R, CHI>(c, X < R(0) ? -X : X);
+    if constexpr (CHI != 0) {
+        const R ax = X < R(0) ? -X : X;
+        if (!NEAR || ax >= R(3)) lor *= chi_sel<R, CHI>(c, ax);
+    }
     return lor;
 }
 
@@ -287,12 +292,12 @@ __device__ inline R doppler(R X, const LineShape<R> &p) {
 
 // Non-Lorentz part of voigt(X): the region is decided from FP64 VX/VY formed exactly as
 // the FP64 form forms them (src/Shapes.f90:49-50), the value in precision R.
-template <typename R>
+template <typename R, bool NEAR = false>
 __device__ inline R voigt_regions(double X64, const LineShape<R> &p, const ShapeCtl &c) {
     const double VX = fabs(X64) * p.xs64;
     const double VY = p.vy64;
     const R X = (R)X64;
-    if (VX >= 15.0) return chilor<R>(X, p, c);
+    if (VX >= 15.0) return chilor<R, -1, NEAR>(X, p, c);
     int region;
     if (VX + VY >= 15.0) region = 1;
     else if (VX + VY >= 5.5) region = 2;
@@ -304,7 +309,7 @@ __device__ inline R voigt_regions(double X64, const LineShape<R> &p, const Shape
     }
     if (VX > 5.0) {
         R vx = (R)VX;
-        return chilor<R>(X, p, c) * (R(1) + fdiv<R>(R(1.5f), vx * vx));
+        return chilor<R, -1, NEAR>(X, p, c) * (R(1) + fdiv<R>(R(1.5f), vx * vx));
     }
     if (VX > (double)MARFA_SQRT14_F) {
         // table index decided in FP64 as the FP64 form decides it
@@ -341,12 +346,12 @@ __device__ __noinline__ R voigt_slow(double X64, double xs64, double vy64, R lor
 template <typename R, bool NEAR = false, int CHI = -1>
 __device__ inline R shape_eval(double X64, const LineShape<R> &p, const ShapeCtl &c) {
     const R X = (R)X64;
-    if (c.shape_id == 1) return chilor<R, CHI>(X, p, c);  // lorentzShape
+    if (c.shape_id == 1) return chilor<R, CHI, NEAR>(X, p, c);  // lorentzShape
     const R ax = X < R(0) ? -X : X;
     // fast path: clearly in region 0 (pure Lorentz x chi); margin covers the FP32 error of VX
     const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
-    if (ax * p.xs >= margin) return chilor<R, CHI>(X, p, c);
-    if constexpr (NEAR && sizeof(R) == 4) return voigt_regions<R>(X64, p, c);
+    if (ax * p.xs >= margin) return chilor<R, CHI, NEAR>(X, p, c);
+    if constexpr (NEAR && sizeof(R) == 4) return voigt_regions<R, true>(X64, p, c);
     else return voigt_slow<R>(X64, p.xs64, p.vy64, p.lorc, p.gl2, p.normc, p.dopa, p.igd, c);
 }
 
