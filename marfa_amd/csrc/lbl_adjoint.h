@@ -95,11 +95,39 @@ struct LineAdj {
     // Such a line (lw: left of the subinterval): its 21 positions at |X| = dmin + q/2 (left)
     // or dmin + (20 - q)/2 (right), weighted by the record's position adjoints gp, summed
     // over the record in precision R and added to the FP64 sums once.
+    // CHIF false: the job has no chi (the caller's launch), and the three sums reduce to
+    // A = sum g/d, B = sum g/d^2, C = sum g |X|/d^2 (d = X^2 + gL^2): sample sum lorc A,
+    // d/d|X| -2 lorc C, d/dgL S/pi (X^2 - gL^2)/d^2 summed = S/pi (A - 2 gL^2 B).
+    template <bool CHIF = true>
     __device__ inline void far(double dmin, bool lw) {
         const float4 *g4 = reinterpret_cast<const float4 *>(gp);  // GPOS floats, 16-byte aligned
         const R base = lw ? (R)dmin : (R)(dmin + 10.0), step = lw ? R(0.5) : R(-0.5);
-        const bool chi = ctl->mol == 2 && ctl->chi_id != 0;
         const R spi = p->S / R(MARFA_PI_F), gl2 = p->gl2, lorc = p->lorc;
+        if constexpr (!CHIF) {
+            R A = R(0), B = R(0), C = R(0);
+#pragma unroll 1
+            for (int q0 = 0; q0 <= 20; q0 += 4) {
+                const float4 x = g4[q0 >> 2];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int q = q0 + e;
+                    if (q > 20) break;
+                    const R gs = (R)(e == 0 ? x.x : e == 1 ? x.y : e == 2 ? x.z : x.w);
+                    const R ax = fma(step, (R)q, base);
+                    const R inv = recip<R>(ax * ax + gl2);
+                    const R gi = gs * inv, gi2 = gi * inv;
+                    A += gi;
+                    B += gi2;
+                    C = fma(gi2, ax, C);
+                }
+            }
+            const double sX = -2.0 * (double)lorc * (double)C;
+            aS += (double)lorc * (double)A;
+            aX += lw ? -sX : sX;  // X = -|X| on the left wing
+            aL += (double)spi * ((double)A - 2.0 * (double)gl2 * (double)B);
+            return;
+        }
+        const bool chi = ctl->mol == 2 && ctl->chi_id != 0;
         R sS = R(0), sX = R(0), sL = R(0), sT = R(0);
         // positions four at a time (one 16-byte load of their adjoints)
 #pragma unroll 1
@@ -227,7 +255,7 @@ struct LineAdj {
 // (centre lines, wings within 1 cm-1, far wings off that branch); launched after pass 0, its
 // thread adds to the slots the same line's pass-0 thread wrote and writes its own level
 // partials.  The chain rule is linear in the sums, so each pass applies it to its own.
-template <typename R, int PASS>
+template <typename R, int PASS, bool CHIF = true>
 __global__ __launch_bounds__(BLOCK, PASS == 0 ? 4 : sizeof(R) == 4 ? 3 : 2) void adjoint_line_kernel(KArgs a, AdjArgs b) {
     const int lev = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -264,7 +292,7 @@ __global__ __launch_bounds__(BLOCK, PASS == 0 ? 4 : sizeof(R) == 4 ? 3 : 2) void
                 // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
                 if (lw ? d > cut : d >= cut) continue;
                 const bool fw = d >= 1.0 && w.far_branch(d);
-                if (PASS == 0 && fw) w.far(d, lw);
+                if (PASS == 0 && fw) w.template far<CHIF>(d, lw);
                 if (PASS == 1 && !fw) {
                     if (lw) w.left();
                     else w.right();

@@ -204,8 +204,13 @@ template <typename R> __device__ inline Dual2<R> operator*(Dual2<R> x, Dual2<R> 
     return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b};
 }
 template <typename R> __device__ inline Dual2<R> operator/(Dual2<R> x, Dual2<R> y) {
-    const R q = x.v / y.v;
-    return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
+    if constexpr (sizeof(R) == 4) {  // FP32: one hardware reciprocal for the three quotients
+        const R r = recip<R>(y.v), q = x.v * r;
+        return {q, (x.a - q * y.a) * r, (x.b - q * y.b) * r};
+    } else {
+        const R q = x.v / y.v;
+        return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
+    }
 }
 
 // Humlicek-Kuntz K(x,y) rational regions 1-3 (src/Shapes.f90:60-117), VY = y, V2 = x^2.
@@ -380,7 +385,7 @@ struct SampleGrad {
 template <typename R>
 __device__ inline void chilor_grad(R X, const LineShape<R> &p, const ShapeCtl &c, SampleGrad<R> &o) {
     const R ax = X < R(0) ? -X : X, sg = X < R(0) ? R(-1) : R(1);
-    const R inv = R(1) / (X * X + p.gl2);
+    const R inv = recip<R>(X * X + p.gl2);
     const R lor = p.lorc * inv;
     R chi = R(1), dchi = R(0);
     if (c.mol == 2 && c.chi_id != 0) {
@@ -441,7 +446,7 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
     }
     if (VX > 5.0) {  // asymptote 1: Lorentz x chi x (1 + 1.5 / VX^2)
         chilor_grad<R>(X, p, c, o);
-        const R f = R(1) + R(1.5f) / (vx * vx), df = R(-3) / (vx * vx * vx);
+        const R f = R(1) + fdiv<R>(R(1.5f), vx * vx), df = fdiv<R>(R(-3), vx * vx * vx);
         o.dX = o.dX * f + o.v * df * p.xs * sg;
         o.dgD = o.v * df * (-vx * p.igd);
         o.dgL = o.dgL * f;
@@ -459,10 +464,10 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
         const int I = (int)(VX / 0.5 - (double)1.00001f);
         const R F = R(2) * (R(ASYM_W[I - 1]) * (R(ASYM_U[I]) - vx) + R(ASYM_W[I]) * (vx - R(ASYM_U[I - 1])));
         const R dF = R(2) * (R(ASYM_W[I]) - R(ASYM_W[I - 1]));
-        const R ix2 = R(1) / (X * X);
+        const R ix2 = recip<R>(X * X);
         const R T2 = p.lorc * ix2 * (R(1) + F);
         o.v += T2;
-        o.dX += R(-2) * T2 / X + p.lorc * ix2 * dF * p.xs * sg;
+        o.dX += fdiv<R>(R(-2) * T2, X) + p.lorc * ix2 * dF * p.xs * sg;
         o.dgD += p.lorc * ix2 * dF * (-vx * p.igd);
         o.dgL = p.S / R(MARFA_PI_F) * ix2 * (R(1) + F);
     }

@@ -1362,11 +1362,15 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
             ab.gpos = (const float *)p_gpos;
             ab.gline = (double *)p_gline;
             ab.glev = (double *)p_glev;
+            // pass 0 (far wings) without the chi terms when the job has no chi
+            const bool chif = job.mol_code == 2 && job.chi_id != 0;
             if (job.precision == MARFA_FP32K) {
-                hipLaunchKernelGGL((adjoint_line_kernel<float, 0>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+                if (chif) hipLaunchKernelGGL((adjoint_line_kernel<float, 0, true>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+                else hipLaunchKernelGGL((adjoint_line_kernel<float, 0, false>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
                 hipLaunchKernelGGL((adjoint_line_kernel<float, 1>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
             } else {
-                hipLaunchKernelGGL((adjoint_line_kernel<double, 0>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+                if (chif) hipLaunchKernelGGL((adjoint_line_kernel<double, 0, true>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
+                else hipLaunchKernelGGL((adjoint_line_kernel<double, 0, false>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
                 hipLaunchKernelGGL((adjoint_line_kernel<double, 1>), dim3(nblk, nb), dim3(BLOCK), 0, st, ka, ab);
             }
             HIPCHK(hipGetLastError());
