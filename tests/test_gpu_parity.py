@@ -74,6 +74,10 @@ GRID_CASES = {
     # centre lines per tile than the LDS stage holds: lines outside a record's window lie
     # between window lines in shifted order in the global-memory lane passes
     "shifted": dict(seed=14, n=45000, lo=3995.0, hi=4025.0, a=4000.0, b=4020.0, cut=0.5, nlev=2),
+    # far wings in every Tonkov band (3-150, 150-300, beyond 300 cm-1) and across its band
+    # edges; Perrin's temperature-dependent bands (edges 30 and 120 cm-1) per node
+    "chi1far": dict(seed=15, n=5000, lo=3670.0, hi=4360.0, a=4000.0, b=4020.0, cut=330.0, nlev=2, chi=1),
+    "chi3far": dict(seed=16, n=3000, lo=3840.0, hi=4190.0, a=4000.0, b=4020.0, cut=160.0, nlev=2, chi=3),
 }
 
 
