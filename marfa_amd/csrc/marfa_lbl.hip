@@ -230,6 +230,7 @@ __device__ inline bool level0_masks(double nus, double org, double cut, uint32_t
 template <typename R, int CHI>
 __device__ inline int chi_band(R ax) {
     if constexpr (CHI == 1) return ax > R(3) ? (ax <= R(150) ? 1 : ax <= R(300) ? 2 : 3) : 0;
+    else if constexpr (CHI == 3) return ax > R(3) ? (ax < R(30) ? 1 : ax < R(120) ? 2 : 3) : 0;
     else return ax >= R(3) ? (ax < R(10) ? 1 : 2) : 0;
 }
 template <typename R, int CHI>
@@ -240,6 +241,18 @@ __device__ inline void chi_law(int b, R &coef, R &rate) {
     } else {
         coef = b == 0 ? R(1) : b == 1 ? R(1.35f) : R(0.614f);
         rate = b == 0 ? R(0) : b == 1 ? R(0.1) : R(1.0 / 47.0);
+    }
+}
+
+// the exponent's slope of chi in band b (Perrin, src/ChiFactors.f90:82-122: the level's
+// temperature coefficients B1, B2 and the constant 0.0232)
+template <typename R, int CHI>
+__device__ inline R chi_rate(int b, const ShapeCtl &c) {
+    if constexpr (CHI == 3) return b <= 1 ? (R)c.B1 : b == 2 ? (R)c.B2 : R(0.0232f);
+    else {
+        R coef, rate;
+        chi_law<R, CHI>(b, coef, rate);
+        return rate;
     }
 }
 
@@ -299,7 +312,7 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
 #pragma unroll 1
                 for (int e = 0; e < (five ? 5 : 3); e++) {
                     const double m = (C + (double)e) * hq;
-                    const R x = shape_eval<R, false, CHI>(plus ? B + m : B - m, p, ctl);
+                    const R x = shape_eval<R, true, CHI>(plus ? B + m : B - m, p, ctl);
                     if (e == 0) v[0] = x;
                     else if (e == 1) v[1] = x;
                     else if (e == 2) {
@@ -384,8 +397,12 @@ __device__ inline void far_run(float base, float step, float gl2, float F, float
 // subinterval).
 template <typename R, int CHI>
 __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
-    constexpr bool kBands = CHI == 1 || CHI == 2;
     constexpr bool kRuns = sizeof(R) == 4;
+    // chi laws whose bands read coef exp(-rate |X|): Tonkov and Pollack (both forms), Perrin
+    // (FP32 form; its rates are the level's)
+    constexpr bool kBands = CHI == 1 || CHI == 2 || (kRuns && CHI == 3);
+    const int lev = r / a.nrec, s = r % a.nrec;
+    const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
     // FP64 form: exp(-rate_b step), step = +1/2 (even entries), -1/2.  FP32 form: the run
     // keys' node factors 2^-32 exp(-rate_b step q) (key 2b + 0 / 1: band b, left / right
     // wing; the last key: no chi)
@@ -396,11 +413,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         for (int t = threadIdx.x; t < NKEY * 21; t += BLOCK) {
             const int key = t / 21, q = t % 21;
             double f = 0x1p-32;
-            if (key < NKEY - 1) {
-                R coef, rate;
-                chi_law<R, CHI>(key >> 1, coef, rate);
-                f *= exp(-(double)rate * ((key & 1) ? -0.5 : 0.5) * q);
-            }
+            if (key < NKEY - 1) f *= exp(-(double)chi_rate<R, CHI>(key >> 1, ctl) * ((key & 1) ? -0.5 : 0.5) * q);
             node_fac[key][q] = f;
         }
     } else if (kBands && threadIdx.x < 8) {
@@ -409,14 +422,12 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         rho_tab[threadIdx.x] = exp(-(double)rate * ((threadIdx.x & 1) ? -0.5 : 0.5));
     }
     __syncthreads();
-    const int lev = r / a.nrec, s = r % a.nrec;
     const double org = a.org[s];
     const int64_t lo = a.lo[s], hi = a.hi[s];
     // line indices are int32: the window test as one unsigned compare
     const int32_t lo32 = (int32_t)lo;
     const uint32_t span = hi > lo ? (uint32_t)(hi - lo) : 0u;
     const double cut = (double)a.cutoff;
-    const ShapeCtl ctl = make_ctl(a, a.lev0 + lev);
     const bool nochi = CHI == 0 || ctl.mol != 2;
     const LineShape<R> *shp = reinterpret_cast<const LineShape<R> *>(a.shp) + (int64_t)lev * a.nl;
     const int32_t *bst = a.bst + (int64_t)lev * (a.NB + 1);
@@ -489,6 +500,19 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         f.gl2 = q.gl2;
         return f;
     };
+    // FP32 form: does chi keep one band b over the 21 nodes?  |X| is monotone in q, so the
+    // bands of nodes 0 and 20 (as the nodes are formed) decide; Pollack: band 2 from 10 cm-1
+    // on, and its other bands are narrower than the 10 cm-1 the nodes span
+    auto one_band = [&](bool lw, R base, R step, int &b) {
+        const R ax20 = fma(step, R(20), base);
+        if constexpr (CHI == 2) {
+            b = 2;
+            return (lw ? base : ax20) >= R(10);
+        } else {
+            b = chi_band<R, CHI>(base);
+            return b == chi_band<R, CHI>(ax20);
+        }
+    };
     int vslow0 = 0x7fffffff, vslow1 = -1;  // the FP32 form's lines for the second pass
     Far n1{}, n2{};
     if ((int)threadIdx.x < nfar) n1 = far_at(threadIdx.x);
@@ -518,13 +542,17 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
             R F = f.lorc * R(0x1p32f);
             bool one_law = lor && nochi;
             if constexpr (kBands) {
-                // Pollack: |X| >= dmin >= 1 puts the far node 10 cm-1 on in band 2, so one band
-                // means band 2 at the nearest node
-                const int b = CHI == 2 ? ((R)dmin >= R(10) ? 2 : -1) : chi_band<R, CHI>((R)dmin);
-                if (lor && !nochi && (CHI == 2 ? b == 2 : b == chi_band<R, CHI>((R)(dmin + 10.0)))) {
-                    R coef, rate;
-                    chi_law<R, CHI>(CHI == 2 ? 2 : b, coef, rate);
-                    F = f.lorc * (coef * R(0x1p32f) * dexp<R>(-(rate * base)));
+                int b;
+                if (lor && !nochi && one_band(lw, base, step, b)) {
+                    R chi0;
+                    if constexpr (CHI == 3) {
+                        chi0 = chi_factor<R>(3, base, ctl.B1, ctl.B2);
+                    } else {
+                        R coef, rate;
+                        chi_law<R, CHI>(b, coef, rate);
+                        chi0 = coef * dexp<R>(-(rate * base));
+                    }
+                    F = f.lorc * (R(0x1p32f) * chi0);
                     key = 2 * b + (lw ? 0 : 1);
                     one_law = true;
                 }
@@ -547,7 +575,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
                 far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { pos[q].add((double)inv * lorc); });
                 continue;
             }
-            if constexpr (kBands) {
+            if constexpr (kBands && !kRuns) {
                 const int b = chi_band<R, CHI>((R)dmin);
                 if (b == chi_band<R, CHI>((R)(dmin + 10.0))) {
                     R coef, rate;
@@ -593,7 +621,8 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
             if (lor) {
                 if (nochi) continue;
                 if constexpr (kBands) {
-                    if (chi_band<R, CHI>((R)dmin) == chi_band<R, CHI>((R)(dmin + 10.0))) continue;
+                    int b;
+                    if (one_band(lw, base, step, b)) continue;
                 }
                 LineShape<R> pl;  // chilor_abs reads the Lorentz fields only
                 pl.lorc = f.lorc;
@@ -619,13 +648,58 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         aM[c] = pos[2 * c + 1];
         aL[c] = pos[2 * c + 2];
     }
-    // Grids 1..5 of the record come from the same near lines (centre lines and the wings
-    // within 1 cm-1 of an edge): per wavefront, partial node sums in LDS, added to by the
-    // last lane of each pair's segment after a lane scan (near_grids_lanes); the
-    // wavefronts' partials are summed in wave order at the end.
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // near lines (centre lines and the wings within 1 cm-1 of an edge), first pass: their
+    // masked grid-0 samples into the 31 node sums, one line per thread
+    for (int j = Ja + (int)threadIdx.x; j < Jb; j += BLOCK) {
+        const LineShape<R> p = shp[j];
+        if (p.idx < lo || p.idx >= hi) continue;
+        const double nus = p.nus;
+        const double u = nus - org;
+        uint32_t mP = 0, mM = 0, mL = 0;
+        if (!level0_masks(nus, org, cut, mP, mM, mL)) continue;
+        const uint32_t need = mP | mM | mL | 1u;
+#pragma unroll
+        for (int q = 0; q <= 20; q++) {
+            // positions no role needs are moved far out, onto the cheap Lorentz branch
+            const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
+            const double val = (double)shape_eval<R, false, CHI>(X, p, ctl);
+            if (q == 0) aR.add(val);
+            if (q & 1) {
+                aM[q >> 1].add(((mM >> q) & 1u) ? val : 0.0);
+            } else {
+                if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : 0.0);
+                if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : 0.0);
+            }
+        }
+    }
+
+    // fixed-order reduction of the node sums: wave butterflies, then the 4 waves in order,
+    // in FP64
+    __shared__ double red[BLOCK / 64][31];
+    {
+        double vals[31];
+        vals[0] = aR.value();
+#pragma unroll
+        for (int c = 0; c < 10; c++) {
+            vals[1 + c] = aP[c].value();
+            vals[11 + c] = aM[c].value();
+            vals[21 + c] = aL[c].value();
+        }
+#pragma unroll
+        for (int k = 0; k < 31; k++) {
+            double x = vals[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if (lane == 0) red[wv][k] = x;
+        }
+    }
+
+    // Second pass: grids 1..5 of the record from the same near lines, one line per lane
+    // (near_grids_lanes, the Voigt regions inlined: no node sums are live here); per
+    // wavefront, partial node sums in LDS, added to by the last lane of each pair's segment
+    // after a lane scan; the wavefronts' partials are summed in wave order at the end.
     __shared__ double g13[BLOCK / 64][NG13];
     __shared__ R g45[BLOCK / 64][NG45];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int e = lane; e < NG13; e += 64) g13[wv][e] = 0.0;
     for (int e = lane; e < NG45; e += 64) g45[wv][e] = R(0);
     LineShape<R> nxt;  // the next centre line's record, loaded while the current one is evaluated
@@ -637,44 +711,11 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         if (j + BLOCK < Jb) nxt = shp[j + BLOCK];
         bool ok = j < Jb && !(p.idx < lo || p.idx >= hi);
         const double nus = p.nus;
-        const double u = nus - org;
         uint32_t mP = 0, mM = 0, mL = 0;
         if (ok) ok = level0_masks(nus, org, cut, mP, mM, mL);
-        if (ok) {
-            const uint32_t need = mP | mM | mL | 1u;
-#pragma unroll
-            for (int q = 0; q <= 20; q++) {
-                // positions no role needs are moved far out, onto the cheap Lorentz branch
-                const double X = ((need >> q) & 1u) ? u - 0.5 * q : 1.0e4;
-                const double val = (double)shape_eval<R, false, CHI>(X, p, ctl);
-                if (q == 0) aR.add(val);
-                if (q & 1) {
-                    aM[q >> 1].add(((mM >> q) & 1u) ? val : 0.0);
-                } else {
-                    if (q <= 18) aP[q >> 1].add(((mP >> q) & 1u) ? val : 0.0);
-                    if (q >= 2) aL[(q >> 1) - 1].add(((mL >> q) & 1u) ? val : 0.0);
-                }
-            }
-        }
         near_grids_lanes<R, CHI>(ok, nus, org, p, ctl, g13[wv], g45[wv]);
     }
 
-    // fixed-order reduction: wave butterflies, then the 4 waves in order, in FP64
-    __shared__ double red[BLOCK / 64][31];
-    double vals[31];
-    vals[0] = aR.value();
-#pragma unroll
-    for (int c = 0; c < 10; c++) {
-        vals[1 + c] = aP[c].value();
-        vals[11 + c] = aM[c].value();
-        vals[21 + c] = aL[c].value();
-    }
-#pragma unroll
-    for (int k = 0; k < 31; k++) {
-        double x = vals[k];
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        if (lane == 0) red[wv][k] = x;
-    }
     __syncthreads();
     if (threadIdx.x < 31) {
         double x = red[0][threadIdx.x];
@@ -695,7 +736,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
 
 // One job's level batch: one workgroup per record, each XCD a contiguous range of records.
 template <typename R, int CHI>
-__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 3 : 2) void coarse_kernel(KArgs a) {
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 2) void coarse_kernel(KArgs a) {
     const int nitems = a.nlev * a.nrec;
     const int r = xcd_item(nitems);
     if (r >= nitems) return;
