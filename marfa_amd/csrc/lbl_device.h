@@ -119,6 +119,23 @@ struct LineShape {
 // deposit kernels than this layout.
 static_assert(sizeof(LineShape<float>) == 72 && sizeof(LineShape<double>) == 104, "line record layout");
 
+// The fields the level-0 kernel's far wings read (FP32 form), 24 B per sorted line, written
+// by the sort beside the line records of a level batch ([levels][lines] after the records):
+// 64 lanes' far lines take 1.5 KB in two loads instead of three loads across 4.6 KB of
+// records.
+struct FarRec {
+    double nus;
+    int32_t idx;
+    float xs, lorc, gl2;
+};
+static_assert(sizeof(FarRec) == 24, "far record layout");
+
+// bytes per sorted line of a level batch's line records (FP32 form: with the far records)
+template <typename R>
+constexpr size_t line_record_bytes() {
+    return sizeof(LineShape<R>) + (sizeof(R) == 4 ? sizeof(FarRec) : 0);
+}
+
 template <typename R>
 __device__ inline LineShape<R> make_line(double gD, double gL, double S) {
     LineShape<R> p;

@@ -242,6 +242,16 @@ __global__ __launch_bounds__(BLOCK) void order_shape_kernel(const uint64_t *keys
         LineShape<R> p = make_line<R>(gD[i], gL[i], S[i] * sscale[lev0 + lev]);
         p.nus = nus[i];
         p.idx = x;
+        if constexpr (sizeof(R) == 4) {
+            // the far record, after the batch's gridDim.y levels of line records
+            FarRec f;
+            f.nus = p.nus;
+            f.idx = x;
+            f.xs = p.xs;
+            f.lorc = p.lorc;
+            f.gl2 = p.gl2;
+            reinterpret_cast<FarRec *>(shp + (int64_t)gridDim.y * nl)[o + q] = f;
+        }
         if (q >= j0 && q < j0 + BLOCK) {
             *(LineShape<R> *)(rec + (q - j0) * (W / 2)) = p;
             filled[q - j0] = 1;

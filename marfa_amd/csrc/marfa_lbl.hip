@@ -490,14 +490,26 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         int32_t idx;
         R xs, lorc, gl2;
     };
+    // FP32 form: the far records after the batch's line records
+    const FarRec *farr = reinterpret_cast<const FarRec *>(reinterpret_cast<const LineShape<R> *>(a.shp) +
+                                                          (int64_t)a.nlev * a.nl) + (int64_t)lev * a.nl;
     auto far_at = [&](int v) {
         Far f;
-        const LineShape<R> &q = shp[far_j(v)];
-        f.nus = q.nus;
-        f.idx = q.idx;
-        f.xs = q.xs;
-        f.lorc = q.lorc;
-        f.gl2 = q.gl2;
+        if constexpr (kRuns) {
+            const FarRec q = farr[far_j(v)];
+            f.nus = q.nus;
+            f.idx = q.idx;
+            f.xs = q.xs;
+            f.lorc = q.lorc;
+            f.gl2 = q.gl2;
+        } else {
+            const LineShape<R> &q = shp[far_j(v)];
+            f.nus = q.nus;
+            f.idx = q.idx;
+            f.xs = q.xs;
+            f.lorc = q.lorc;
+            f.gl2 = q.gl2;
+        }
         return f;
     };
     // FP32 form: does chi keep one band b over the 21 nodes?  |X| is monotone in q, so the
@@ -1236,7 +1248,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     geom_of(job, W, G0);
     const double bbase = G0.bbase;
     const int NB = G0.NB;
-    const size_t shp_bytes = job.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
+    const size_t shp_bytes = job.precision == MARFA_FP32K ? line_record_bytes<float>() : line_record_bytes<double>();
 
     const int nsblk = G0.nsblk;
     const unsigned nblk = G0.nblk;
@@ -2105,7 +2117,7 @@ int plan_waves(marfa_batch *b) {
         const auto &J = b->jobs[j];
         if (!geom_of(J.spec, J.W, geo[j])) continue;
         const RunGeom &g = geo[j];
-        const size_t shp = J.spec.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
+        const size_t shp = J.spec.precision == MARFA_FP32K ? line_record_bytes<float>() : line_record_bytes<double>();
         need[j].keep = (size_t)g.nl * (8 + 4 + shp) + (size_t)(g.NB + 1) * 4 + (size_t)g.nrec * NCOARSE * 8 + 5 * 256;
         need[j].temp = (size_t)g.nl * (4 * 8 + 8 + 4) + (size_t)g.NB * 4 + (size_t)g.nsblk * 4;
         const int nlev = (int)J.L.P.size();
@@ -2160,7 +2172,7 @@ int plan_waves(marfa_batch *b) {
             const auto &J = b->jobs[P.job];
             const RunGeom &g = geo[P.job];
             const size_t arr = (size_t)P.nb * g.nl;
-            const size_t shp = J.spec.precision == MARFA_FP32K ? sizeof(LineShape<float>) : sizeof(LineShape<double>);
+            const size_t shp = J.spec.precision == MARFA_FP32K ? line_record_bytes<float>() : line_record_bytes<double>();
             P.B.key0 = ar.take(arr * 8);
             P.B.idx0 = ar.take(arr * 4);
             P.B.shapes = ar.take(arr * shp);
