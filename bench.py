@@ -483,6 +483,8 @@ def main():
             "flop_model": kern[dom]["flop_model"] + " per launch",
             "kernels": {k: {"ms": v["ms"], "tflops": v["tflops"], "frac": v["frac"]} for k, v in kern.items()}}
     whole = (FLOP_PER_SAMPLE * samples + FLOP_PER_RECORD_CASCADE * records) / dt / 1e12
+    # the whole step against the same model (every sample and cascade over the step's time)
+    roof["step"] = {"tflops": whole, "frac": whole / PEAK[dt_name]}
 
     result = {
         "metric": "PT-table spectral pts/s at <=1e-4 rel err vs FP64; fwd+adjoint ms per level",

@@ -279,7 +279,24 @@ __global__ __launch_bounds__(BLOCK, PASS == 0 ? 4 : sizeof(R) == 4 ? 3 : 2) void
             else r = mid + 1;
         }
         const double cut = (double)a.cutoff;
+        // pass 1 works only where the line is no far wing on the Lorentz branch: a wing at
+        // d >= dfar is one (d >= 1 and (R)d xs >= the branch margin, with a relative 1e-5 to
+        // spare over the rounding of (R)d xs), so only the records with origins in
+        // (nus - 10 - dfar, nus + dfar) are visited (subinterval origins increase with r)
+        double dfar = 1.0;
+        if (PASS == 1) {
+            const R margin = sizeof(R) == 8 ? R(15.0 * (1.0 + 1e-12)) : R(15.0f * (1.0f + 4e-6f));
+            if (ctl.shape_id != 1) dfar = fmax(1.0, (double)margin / (double)p.xs * (1.0 + 1e-5));
+            const double o0 = nus - 10.0 - dfar;
+            int hiO = a.nrec;
+            while (r < hiO) {
+                const int mid = (r + hiO) >> 1;
+                if (a.org[mid] > o0) hiO = mid;
+                else r = mid + 1;
+            }
+        }
         for (; r < a.nrec && a.lo[r] <= i; r++) {
+            if (PASS == 1 && a.org[r] >= nus + dfar) break;
             if (i >= a.hi[r]) continue;
             const double org = a.org[r];
             const int64_t rr = (int64_t)lev * a.nrec + r;
