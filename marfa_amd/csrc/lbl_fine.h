@@ -647,12 +647,29 @@ __device__ inline double cascade_a2(double P, double M, double L, bool &lim) {
     return lim ? 0.5 * (L + M) : A;
 }
 
+// Limiter choices of a tile's cascade, kept for the adjoint's transposed cascade: the step
+// from grid k (k = 1..9) to the next puts the two flags of its cell c (A1, A2 limited) at
+// bits lim_bit(k, c) and lim_bit(k, c) + 1 -- 2,044 bits in LIMW words per tile.
+constexpr int LIMW = 64;
+__device__ inline int lim_bit(int k, int c) { return (1 << (k + 1)) - 4 + 2 * c; }
+__device__ inline void lim_set(uint32_t *slim, int k, int c, bool l1, bool l2) {
+    const int b = lim_bit(k, c);
+    const uint32_t v = ((uint32_t)l1 | ((uint32_t)l2 << 1)) << (b & 31);
+    if (v) atomicOr(&slim[b >> 5], v);  // LDS: OR is order-independent
+}
+__device__ inline void lim_get(const uint32_t *slim, int k, int c, bool &l1, bool &l2) {
+    const int b = lim_bit(k, c);
+    const uint32_t v = slim[b >> 5] >> (b & 31);
+    l1 = v & 1u;
+    l2 = (v >> 1) & 1u;
+}
+
 // Cascade of tile t in FP64 from its grid-0 cell (cz: P, M, L of the 10 cells) down to
 // grid 9 (src/Grids.f90:104-139): afterwards g holds the cascaded values of every level.
-__device__ inline void tile_cascade(double *g, const double *cz, int t) {
+// slim (LDS, zeroed, or null): the limiter choices of grids 1..8 are recorded there.
+__device__ inline void tile_cascade(double *g, const double *cz, int t, uint32_t *slim = nullptr) {
     const int tid = threadIdx.x;
-    auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i) {
-        bool l1, l2;
+    auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i, bool &l1, bool &l2) {
         fP[i] += P;
         fM[i] += cascade_a1(P, M, L, l1);
         fL[i] += M;
@@ -662,13 +679,17 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t) {
     };
     if (tid == 0) {
         const int k1 = lvl_acc_off(1);
-        step(cz[1 + t], cz[11 + t], cz[21 + t], g + k1, g + k1 + 2, g + k1 + 4, 0);
+        bool l1, l2;
+        step(cz[1 + t], cz[11 + t], cz[21 + t], g + k1, g + k1 + 2, g + k1 + 4, 0, l1, l2);
     }
     __syncthreads();
     for (int k = 1; k < 9; k++) {
         const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
-        for (int c = tid; c < nk; c += BLOCK)
-            step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c);
+        for (int c = tid; c < nk; c += BLOCK) {
+            bool l1, l2;
+            step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c, l1, l2);
+            if (slim) lim_set(slim, k, c, l1, l2);
+        }
         __syncthreads();
     }
 }
@@ -682,6 +703,10 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
     const int lev = r / a.nrec, s = r % a.nrec;
     const double sc = a.sscale[a.lev0 + lev];
     const int tid = threadIdx.x;
+    // the cascade's limiter choices, when the caller keeps them for the adjoint
+    __shared__ uint32_t s_lim[LIMW];
+    uint32_t *slim = a.lim ? s_lim : nullptr;
+    if (slim && tid < LIMW) s_lim[tid] = 0u;  // ordered before their use by the barriers below
     FineCtx<R> f;
     tile_ctx<R>(a, lev, s, f);
     const GlobalSrc<R> gsrc(f);
@@ -699,7 +724,7 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
         if (tid < 3) a.grids[tid * 10 + t] = cz[1 + 10 * tid + t] / sc;
         if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
     }
-    tile_cascade(g, cz, t);
+    tile_cascade(g, cz, t, slim);
 
     // ---- fine points: direct deposits summed in registers (thread owns the 4 points of
     // grid-9 cells tid and tid + 256), then the last cascade step and the record write
@@ -746,6 +771,7 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
         v[1] += M;
         v[2] += cascade_a2(P, M, L, l2);
         v[3] += L;
+        if (slim) lim_set(slim, 9, c, l1, l2);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             if (a.out64) a.out64[ro + 2048 * t + 1 + 4 * c + e] = v[e] * mul;
@@ -755,6 +781,10 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
     if (t == 0 && tid == 0) {
         if (a.out64) a.out64[ro] = cz[0] * mul;
         else a.out[ro] = (float)(cz[0] * mul);
+    }
+    if (slim) {
+        __syncthreads();
+        if (tid < LIMW) a.lim[((int64_t)r * TILES + t) * LIMW + tid] = s_lim[tid];
     }
 }
 
@@ -781,8 +811,10 @@ __global__ __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) void fine_kernel(KAr
 // dL/d(pre-cascade node) in the reference's grid layout (grids 0..9 P|mid|L blocks, then
 // RK(1..20481)), for samples scaled by the level's power-of-two scale.  The tile recomputes
 // its grid 1..9 sums and the cascade (the limiter's choices need the cascaded values), then
-// runs the transposed cascade from its 2,048 points up to its grid-0 cell.
-template <typename R>
+// runs the transposed cascade from its 2,048 points up to its grid-0 cell.  LIM: the forward
+// kept the cascade's limiter choices (a.lim), which are all the transposed cascade needs of
+// the forward values, so the grid sums and the forward cascade are not recomputed.
+template <typename R, bool LIM = false>
 __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const float *gout, float *gbar) {
     __shared__ double g[NACC];
     __shared__ float gb[NACC];            // adjoints of the cascaded values, grids 1..9
@@ -798,14 +830,29 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     const int lev = r / a.nrec, s = r % a.nrec;
     const double sc = a.sscale[a.lev0 + lev];
     const int tid = threadIdx.x;
-    FineCtx<R> f;
-    tile_ctx<R>(a, lev, s, f);
-    const GlobalSrc<R> gsrc(f);
-    CentreLds<R> lsrc;
-    const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
-    tile_level_sums<R>(gsrc, lsrc, staged, t, g, red, cz + NG0);
-    tile_cascade(g, cz, t);
+    __shared__ uint32_t s_lim[LIMW];
+    if constexpr (LIM) {
+        if (tid < LIMW) s_lim[tid] = a.lim[((int64_t)r * TILES + t) * LIMW + tid];
+        __syncthreads();
+    } else {
+        FineCtx<R> f;
+        tile_ctx<R>(a, lev, s, f);
+        const GlobalSrc<R> gsrc(f);
+        CentreLds<R> lsrc;
+        const bool staged = stage_tile<R>(f, t, s_u, s_j, s_b, s_cnt, lsrc);
+        tile_level_sums<R>(gsrc, lsrc, staged, t, g, red, cz + NG0);
+        tile_cascade(g, cz, t);
+    }
+    // the limiter's choices of the step from grid k's cell c (P, M, L: its cascaded values)
+    auto flags = [&](int k, int c, double P, double M, double L, bool &l1, bool &l2) {
+        if constexpr (LIM) {
+            lim_get(s_lim, k, c, l1, l2);
+        } else {
+            cascade_a1(P, M, L, l1);
+            cascade_a2(P, M, L, l2);
+        }
+    };
 
     const double mul = (double)a.outmul[a.lev0 + lev] / sc;
     const float *go = gout + ((int64_t)(a.lev0 + lev) * a.nrec + s) * NT;
@@ -817,10 +864,8 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     {
         const int cb9 = lvl_acc_off(9);
         for (int c = tid; c < 512; c += BLOCK) {
-            const double P = g[cb9 + c], M = g[cb9 + 512 + c], L = g[cb9 + 1024 + c];
             bool l1, l2;
-            cascade_a1(P, M, L, l1);
-            cascade_a2(P, M, L, l2);
+            flags(9, c, g[cb9 + c], g[cb9 + 512 + c], g[cb9 + 1024 + c], l1, l2);
             const int p0 = 2048 * t + 1 + 4 * c;
             const double r0 = go[p0] * mul, r1 = go[p0 + 1] * mul, r2 = go[p0 + 2] * mul, r3 = go[p0 + 3] * mul;
             gb[cb9 + c] = (float)(r0 * (l1 ? 0.5 : 0.375) + r2 * (l2 ? 0.0 : -0.125));
@@ -830,11 +875,8 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     }
     __syncthreads();
     // transposed steps k -> k+1 for k = 8..1, then grid 0
-    auto back = [&](double P, double M, double L, const float *cP, const float *cM, const float *cL, int i,
+    auto back = [&](bool l1, bool l2, const float *cP, const float *cM, const float *cL, int i,
                     float &oP, float &oM, float &oL) {
-        bool l1, l2;
-        cascade_a1(P, M, L, l1);
-        cascade_a2(P, M, L, l2);
         const double m0 = cM[i], m1 = cM[i + 1];
         oP = (float)(cP[i] + m0 * (l1 ? 0.5 : 0.375) + m1 * (l2 ? 0.0 : -0.125));
         oM = (float)(m0 * (l1 ? 0.5 : 0.75) + cL[i] + cP[i + 1] + m1 * (l2 ? 0.5 : 0.75));
@@ -842,15 +884,21 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     };
     for (int k = 8; k >= 1; k--) {
         const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
-        for (int c = tid; c < nk; c += BLOCK)
-            back(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], gb + fb, gb + fb + 2 * nk, gb + fb + 4 * nk, 2 * c,
-                 gb[cb + c], gb[cb + nk + c], gb[cb + 2 * nk + c]);
+        for (int c = tid; c < nk; c += BLOCK) {
+            bool l1, l2;
+            flags(k, c, g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], l1, l2);
+            back(l1, l2, gb + fb, gb + fb + 2 * nk, gb + fb + 4 * nk, 2 * c, gb[cb + c], gb[cb + nk + c],
+                 gb[cb + 2 * nk + c]);
+        }
         __syncthreads();
     }
     if (tid == 0) {
         const int k1 = lvl_acc_off(1);
         float oP, oM, oL;
-        back(cz[1 + t], cz[11 + t], cz[21 + t], gb + k1, gb + k1 + 2, gb + k1 + 4, 0, oP, oM, oL);
+        bool l1, l2;  // the grid-0 cell's step, from the level-0 sums
+        cascade_a1(cz[1 + t], cz[11 + t], cz[21 + t], l1);
+        cascade_a2(cz[1 + t], cz[11 + t], cz[21 + t], l2);
+        back(l1, l2, gb + k1, gb + k1 + 2, gb + k1 + 4, 0, oP, oM, oL);
         gr[t] = oP;
         gr[10 + t] = oM;
         gr[20 + t] = oL;

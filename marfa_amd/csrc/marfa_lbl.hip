@@ -84,6 +84,7 @@ struct KArgs {
     double *out64;                            // same layout in FP64 (instead of out) or null
     double *grids;  // optional: pre-cascade sums of record 0 in the reference's layout
                     // (grids 0..9 as P|mid|L blocks, then RK(1..20481)), sample scale removed
+    uint32_t *lim;  // optional: the cascade's limiter choices, [nlev*nrec][TILES][64 words]
 };
 
 // Buckets of the per-level index of sorted shifted centres (1/512 cm-1 wide).
@@ -914,12 +915,15 @@ struct marfa_ctx {
     double *dbg_grids = nullptr;
     void *fixed_scratch = nullptr;  // this device's `scratch` (marfa_pt_tables_noalloc)
     // Checkpoint of the last forward run: the level-0 kernel's sums (grid 0 and grids 1..5,
-    // NCOARSE values) of every record of every level, kept so that the adjoint of the same
-    // job and parameters skips that kernel.  gen changes whenever lines, levels or TIPS do.
+    // NCOARSE values) of every record of every level, then the fine kernel's cascade limiter
+    // choices (TILES x LIMW words per record), kept so that the adjoint of the same job and
+    // parameters skips the level-0 kernel and the recomputation of the finer grids.  gen
+    // changes whenever lines, levels or TIPS do.
     Buf ckpt;
     uint64_t gen = 0;
     struct CkptKey {
         bool valid = false;
+        bool lim = false;  // the limiter choices beside the sums were written by a forward run
         uint64_t gen = 0;
         marfa_job job{};
         hipStream_t stream = nullptr;
@@ -1208,10 +1212,13 @@ struct AdjReq {
 // every buffer already large enough, so that no allocation, copy or synchronisation occurs.
 // ckpt: device array [nlev][nrec][NCOARSE] for the level-0 sums instead of per-batch scratch
 // (the forward's checkpoint); reuse: it already holds them for this job, so the level-0
-// kernel is not launched.
+// kernel is not launched.  lim: device array [nlev][nrec][TILES][LIMW] beside it, where the
+// forward's fine kernel records the cascade's limiter choices; lim_ok: it holds them for
+// this job, and the adjoint reads them instead of recomputing the grid sums and cascade.
 int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
         hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr, bool capture = false,
-        Arena *arena = nullptr, double *ckpt = nullptr, bool reuse = false) {
+        Arena *arena = nullptr, double *ckpt = nullptr, bool reuse = false, uint32_t *lim = nullptr,
+        bool lim_ok = false) {
     if (capture) timed = false;
     const int nlev = (int)L.P.size(), nrec = W.nrec;
     if (nrec == 0 || nlev == 0) return 0;
@@ -1364,6 +1371,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         KArgs ka;
         int rc = enqueue_prep(c, job, G, D, l0, nb, B, out, out64, st, timed, ka);
         if (rc) return rc;
+        if (lim) ka.lim = lim + (size_t)l0 * nrec * TILES * LIMW;
         if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
         const int nitems = nb * nrec;
         const unsigned gcoarse = (unsigned)(8 * ((nitems + 7) / 8));
@@ -1391,10 +1399,14 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
         if (adj) {
             // adjoint: dL/dnode of every record, then the line pass and its reductions
             float *gbar = (float *)p_gbar;
-            if (job.precision == MARFA_FP32K)
-                hipLaunchKernelGGL(adjoint_tile_kernel<float>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
-            else
-                hipLaunchKernelGGL(adjoint_tile_kernel<double>, dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+            const bool lk = lim && lim_ok;
+            if (job.precision == MARFA_FP32K) {
+                if (lk) hipLaunchKernelGGL((adjoint_tile_kernel<float, true>), dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+                else hipLaunchKernelGGL((adjoint_tile_kernel<float, false>), dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+            } else {
+                if (lk) hipLaunchKernelGGL((adjoint_tile_kernel<double, true>), dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+                else hipLaunchKernelGGL((adjoint_tile_kernel<double, false>), dim3(gfine), dim3(BLOCK), 0, st, ka, adj->gout, gbar);
+            }
             HIPCHK(hipGetLastError());
             hipLaunchKernelGGL(adjoint_gpos_kernel, dim3((unsigned)((nitems * GPOS + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
                                st, (const float *)gbar, nitems, (float *)p_gpos);
@@ -1616,11 +1628,17 @@ static LevelSet level_set(const marfa_ctx *c, const marfa_job &job) {
 static double *ckpt_for(marfa_ctx *c, const Windows &w) {
     c->ck.valid = false;
     if (w.nrec == 0) return nullptr;
-    if (ensure(c, c->ckpt, c->hP.size() * (size_t)w.nrec * NCOARSE * 8)) return nullptr;
+    const size_t recs = c->hP.size() * (size_t)w.nrec;
+    if (ensure(c, c->ckpt, recs * NCOARSE * 8 + recs * TILES * LIMW * 4)) return nullptr;
     return (double *)c->ckpt.p;
 }
-static void ckpt_set(marfa_ctx *c, const marfa_job &job, hipStream_t st, const Windows &w) {
+// the limiter choices beside the level-0 sums in the checkpoint buffer
+static uint32_t *ckpt_lim(marfa_ctx *c, const Windows &w) {
+    return (uint32_t *)((double *)c->ckpt.p + c->hP.size() * (size_t)w.nrec * NCOARSE);
+}
+static void ckpt_set(marfa_ctx *c, const marfa_job &job, hipStream_t st, const Windows &w, bool lim) {
     c->ck.valid = w.nrec > 0;
+    c->ck.lim = lim;
     c->ck.gen = c->gen;
     c->ck.job = job;
     c->ck.stream = st;
@@ -1656,9 +1674,10 @@ int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *o
     // phases are timed (and the host waits for them) unless the call is asynchronous
     double *ck = ckpt_for(c, w);
     if (w.nrec > 0 && !ck) return MARFA_E_HIP;
-    rc = run(c, *job, L, w, out, st, out_host != nullptr || !async, nullptr, nullptr, false, nullptr, ck);
+    rc = run(c, *job, L, w, out, st, out_host != nullptr || !async, nullptr, nullptr, false, nullptr, ck, false,
+             ck ? ckpt_lim(c, w) : nullptr);
     if (rc) return rc;
-    ckpt_set(c, *job, st, w);
+    ckpt_set(c, *job, st, w, true);
     if (out_host) {
         HIPCHK(hipMemcpyAsync(out_host, out, total * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1698,9 +1717,10 @@ int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, voi
     LevelSet L = level_set(c, *job);
     double *ck = ckpt_for(c, w);
     if (w.nrec > 0 && !ck) return MARFA_E_HIP;
-    rc = run(c, *job, L, w, nullptr, st, true, nullptr, out_dev, false, nullptr, ck);
+    rc = run(c, *job, L, w, nullptr, st, true, nullptr, out_dev, false, nullptr, ck, false,
+             ck ? ckpt_lim(c, w) : nullptr);
     if (rc) return rc;
-    ckpt_set(c, *job, st, w);
+    ckpt_set(c, *job, st, w, true);
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -1727,11 +1747,13 @@ static int adjoint_dev(marfa_ctx *c, const marfa_job *job, const float *grad_out
     // line pass) -- the density is held fixed.  The level-0 sums come from the forward's
     // checkpoint when it holds this job with the current inputs, else they are recomputed.
     const bool reuse = ckpt_holds(c, *job, st, w);
+    const bool lim_ok = reuse && c->ck.lim;
     double *ck = reuse ? (double *)c->ckpt.p : ckpt_for(c, w);
     if (!ck) return MARFA_E_HIP;
-    const int rc = run(c, *job, L, w, nullptr, st, timed, &req, nullptr, false, nullptr, ck, reuse);
+    const int rc = run(c, *job, L, w, nullptr, st, timed, &req, nullptr, false, nullptr, ck, reuse,
+                       lim_ok ? ckpt_lim(c, w) : nullptr, lim_ok);
     if (rc) return rc;
-    if (!reuse) ckpt_set(c, *job, st, w);
+    if (!reuse) ckpt_set(c, *job, st, w, false);  // the sums, without limiter choices
     return 0;
 }
 
