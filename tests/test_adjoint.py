@@ -318,9 +318,10 @@ def test_finite_differences_config4():
 
 @pytest.mark.gpu
 def test_adjoint_checkpoint_reuse_and_invalidation():
-    """The adjoint reuses the forward's level-0 sums only for the inputs they were computed
-    from: reused, recomputed after the same parameters are set again, and after a forward at
-    other parameters followed by a return to these (no forward), the gradients are the same."""
+    """The adjoint reuses the forward's level-0 sums and cascade limiter choices only for the
+    inputs they were computed from: reused (no grid sums recomputed), recomputed after the
+    same parameters are set again, and after a forward at other parameters followed by a
+    return to these (no forward), the gradients are the same, bit for bit."""
     from marfa_amd.autograd import PTTableModel
     job = adjoint_job(n=1500, nlev=2)
     model = PTTableModel(job, TIPS)
