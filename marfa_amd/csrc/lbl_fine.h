@@ -647,27 +647,38 @@ __device__ inline double cascade_a2(double P, double M, double L, bool &lim) {
     return lim ? 0.5 * (L + M) : A;
 }
 
-// Limiter choices of a tile's cascade, kept for the adjoint's transposed cascade: the step
-// from grid k (k = 1..9) to the next puts the two flags of its cell c (A1, A2 limited) at
-// bits lim_bit(k, c) and lim_bit(k, c) + 1 -- 2,044 bits in LIMW words per tile.
-constexpr int LIMW = 64;
-__device__ inline int lim_bit(int k, int c) { return (1 << (k + 1)) - 4 + 2 * c; }
-__device__ inline void lim_set(uint32_t *slim, int k, int c, bool l1, bool l2) {
-    const int b = lim_bit(k, c);
-    const uint32_t v = ((uint32_t)l1 | ((uint32_t)l2 << 1)) << (b & 31);
-    if (v) atomicOr(&slim[b >> 5], v);  // LDS: OR is order-independent
+// Limiter choices of a tile's cascade, kept for the adjoint's transposed cascade: for the
+// step from grid k (k = 1..9) to the next, the A1 and A2 flags of its cells in 64-cell
+// groups, one 64-bit word per (group, flag): word lim_word(k, c) + 0 / 1, bit c & 63.
+// Grids 1..6 take one group each, 7 two, 8 four, 9 eight: LIMW = 40 words per tile.  The
+// words are formed by wavefront ballots (a group is one wavefront's cells) and stored by
+// the wavefront's first lane.
+constexpr int LIMW = 40;
+__device__ inline int lim_word(int k, int c) {
+    const int base = k <= 6 ? 2 * (k - 1) : k == 7 ? 12 : k == 8 ? 16 : 24;
+    return base + 2 * (c >> 6);
 }
-__device__ inline void lim_get(const uint32_t *slim, int k, int c, bool &l1, bool &l2) {
-    const int b = lim_bit(k, c);
-    const uint32_t v = slim[b >> 5] >> (b & 31);
-    l1 = v & 1u;
-    l2 = (v >> 1) & 1u;
+// the wavefront's flags of cells c (one per lane, lanes without a cell pass false); the
+// cells of the active lanes share one group, the first active lane being lane 0
+__device__ inline void lim_put(uint64_t *lim, int k, int c, bool l1, bool l2) {
+    const uint64_t w1 = __ballot(l1), w2 = __ballot(l2);
+    if ((threadIdx.x & 63) == 0) {
+        const int w = lim_word(k, c);
+        lim[w] = w1;
+        lim[w + 1] = w2;
+    }
+}
+__device__ inline void lim_get(const uint64_t *lim, int k, int c, bool &l1, bool &l2) {
+    const int w = lim_word(k, c), b = c & 63;
+    l1 = (lim[w] >> b) & 1u;
+    l2 = (lim[w + 1] >> b) & 1u;
 }
 
 // Cascade of tile t in FP64 from its grid-0 cell (cz: P, M, L of the 10 cells) down to
 // grid 9 (src/Grids.f90:104-139): afterwards g holds the cascaded values of every level.
-// slim (LDS, zeroed, or null): the limiter choices of grids 1..8 are recorded there.
-__device__ inline void tile_cascade(double *g, const double *cz, int t, uint32_t *slim = nullptr) {
+// lim (the tile's LIMW words in global memory, or null): the limiter choices of grids 1..8
+// are stored there.
+__device__ inline void tile_cascade(double *g, const double *cz, int t, uint64_t *lim = nullptr) {
     const int tid = threadIdx.x;
     auto step = [&](double P, double M, double L, double *fP, double *fM, double *fL, int i, bool &l1, bool &l2) {
         fP[i] += P;
@@ -685,10 +696,10 @@ __device__ inline void tile_cascade(double *g, const double *cz, int t, uint32_t
     __syncthreads();
     for (int k = 1; k < 9; k++) {
         const int nk = 1 << k, cb = lvl_acc_off(k), fb = lvl_acc_off(k + 1);
-        for (int c = tid; c < nk; c += BLOCK) {
+        for (int c = tid; c < nk; c += BLOCK) {  // nk <= BLOCK: one cell per thread
             bool l1, l2;
             step(g[cb + c], g[cb + nk + c], g[cb + 2 * nk + c], g + fb, g + fb + 2 * nk, g + fb + 4 * nk, 2 * c, l1, l2);
-            if (slim) lim_set(slim, k, c, l1, l2);
+            if (lim) lim_put(lim, k, c, l1, l2);
         }
         __syncthreads();
     }
@@ -704,9 +715,7 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
     const double sc = a.sscale[a.lev0 + lev];
     const int tid = threadIdx.x;
     // the cascade's limiter choices, when the caller keeps them for the adjoint
-    __shared__ uint32_t s_lim[LIMW];
-    uint32_t *slim = a.lim ? s_lim : nullptr;
-    if (slim && tid < LIMW) s_lim[tid] = 0u;  // ordered before their use by the barriers below
+    uint64_t *lim = a.lim ? a.lim + ((int64_t)r * TILES + t) * LIMW : nullptr;
     FineCtx<R> f;
     tile_ctx<R>(a, lev, s, f);
     const GlobalSrc<R> gsrc(f);
@@ -724,7 +733,7 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
         if (tid < 3) a.grids[tid * 10 + t] = cz[1 + 10 * tid + t] / sc;
         if (t == 0 && tid == 0) a.grids[30690] = cz[0] / sc;
     }
-    tile_cascade(g, cz, t, slim);
+    tile_cascade(g, cz, t, lim);
 
     // ---- fine points: direct deposits summed in registers (thread owns the 4 points of
     // grid-9 cells tid and tid + 256), then the last cascade step and the record write
@@ -771,7 +780,7 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
         v[1] += M;
         v[2] += cascade_a2(P, M, L, l2);
         v[3] += L;
-        if (slim) lim_set(slim, 9, c, l1, l2);
+        if (lim) lim_put(lim, 9, c, l1, l2);  // c: lane + 64 x (the wavefront's group)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             if (a.out64) a.out64[ro + 2048 * t + 1 + 4 * c + e] = v[e] * mul;
@@ -781,10 +790,6 @@ __device__ __forceinline__ void fine_tile(const KArgs &a, int w, double *g, doub
     if (t == 0 && tid == 0) {
         if (a.out64) a.out64[ro] = cz[0] * mul;
         else a.out[ro] = (float)(cz[0] * mul);
-    }
-    if (slim) {
-        __syncthreads();
-        if (tid < LIMW) a.lim[((int64_t)r * TILES + t) * LIMW + tid] = s_lim[tid];
     }
 }
 
@@ -831,7 +836,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_tile_kernel(KArgs a, const floa
     const double sc = a.sscale[a.lev0 + lev];
     const int tid = threadIdx.x;
     const double *cz = a.coarse + (int64_t)r * NCOARSE;
-    __shared__ uint32_t s_lim[LIMW];
+    __shared__ uint64_t s_lim[LIMW];
     if constexpr (LIM) {
         if (tid < LIMW) s_lim[tid] = a.lim[((int64_t)r * TILES + t) * LIMW + tid];
         __syncthreads();

@@ -84,7 +84,7 @@ struct KArgs {
     double *out64;                            // same layout in FP64 (instead of out) or null
     double *grids;  // optional: pre-cascade sums of record 0 in the reference's layout
                     // (grids 0..9 as P|mid|L blocks, then RK(1..20481)), sample scale removed
-    uint32_t *lim;  // optional: the cascade's limiter choices, [nlev*nrec][TILES][64 words]
+    uint64_t *lim;  // optional: the cascade's limiter choices, [nlev*nrec][TILES][LIMW words]
 };
 
 // Buckets of the per-level index of sorted shifted centres (1/512 cm-1 wide).
@@ -1217,7 +1217,7 @@ struct AdjReq {
 // this job, and the adjoint reads them instead of recomputing the grid sums and cascade.
 int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W, float *out,
         hipStream_t st, bool timed, AdjReq *adj = nullptr, double *out64 = nullptr, bool capture = false,
-        Arena *arena = nullptr, double *ckpt = nullptr, bool reuse = false, uint32_t *lim = nullptr,
+        Arena *arena = nullptr, double *ckpt = nullptr, bool reuse = false, uint64_t *lim = nullptr,
         bool lim_ok = false) {
     if (capture) timed = false;
     const int nlev = (int)L.P.size(), nrec = W.nrec;
@@ -1629,12 +1629,12 @@ static double *ckpt_for(marfa_ctx *c, const Windows &w) {
     c->ck.valid = false;
     if (w.nrec == 0) return nullptr;
     const size_t recs = c->hP.size() * (size_t)w.nrec;
-    if (ensure(c, c->ckpt, recs * NCOARSE * 8 + recs * TILES * LIMW * 4)) return nullptr;
+    if (ensure(c, c->ckpt, recs * NCOARSE * 8 + recs * TILES * LIMW * 8)) return nullptr;
     return (double *)c->ckpt.p;
 }
 // the limiter choices beside the level-0 sums in the checkpoint buffer
-static uint32_t *ckpt_lim(marfa_ctx *c, const Windows &w) {
-    return (uint32_t *)((double *)c->ckpt.p + c->hP.size() * (size_t)w.nrec * NCOARSE);
+static uint64_t *ckpt_lim(marfa_ctx *c, const Windows &w) {
+    return (uint64_t *)((double *)c->ckpt.p + c->hP.size() * (size_t)w.nrec * NCOARSE);
 }
 static void ckpt_set(marfa_ctx *c, const marfa_job &job, hipStream_t st, const Windows &w, bool lim) {
     c->ck.valid = w.nrec > 0;
