@@ -352,13 +352,24 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
 }
 
 // The 21 grid-0 nodes of a far wing in the FP64 form, |X_q| = base + q step, in order:
-// use(q, 1 / (X_q^2 + gL^2)).
+// use(q, 1 / (X_q^2 + gL^2)), with one division per pair of nodes (1/a = b / (ab): a few
+// ulps instead of a half; the denominators lie in [1, ~1e5], so the product cannot
+// overflow).
 template <typename R, typename F>
 __device__ inline void far_nodes(R base, R step, R gl2, F use) {
 #pragma unroll
-    for (int q = 0; q <= 20; q++) {
-        const R ax = q == 0 ? base : fma(step, (R)q, base);
-        use(q, recip<R>(ax * ax + gl2));
+    for (int q = 0; q <= 20; q += 2) {
+        const R ax0 = q == 0 ? base : fma(step, (R)q, base);
+        const R d0 = ax0 * ax0 + gl2;
+        if (q == 20) {
+            use(q, recip<R>(d0));
+        } else {
+            const R ax1 = fma(step, (R)(q + 1), base);
+            const R d1 = ax1 * ax1 + gl2;
+            const R r = recip<R>(d0 * d1);
+            use(q, d1 * r);
+            use(q + 1, d0 * r);
+        }
     }
 }
 
