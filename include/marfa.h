@@ -144,8 +144,10 @@ int marfa_pt_tables_f64(marfa_ctx *ctx, const marfa_job *job, double *out_dev, v
  * dL/ddelta; grad_levels (host, [2][nlev]): dL/dT, dL/dP per level (density held fixed,
  * pSelf = rho 10 k T / p0 follows T).  The deposit geometry (which nodes each line samples)
  * is held fixed: the derivative is that of the piecewise-smooth model between its
- * decisions.  The level's grid sums are recomputed (not stored by the forward call).
- * The Perrin chi factor's temperature dependence enters dL/dT. */
+ * decisions.  The level-0 sums kept by the context's last forward of the same job, inputs
+ * and stream are reused (and the cascade's limiter choices, when that forward kept them:
+ * marfa_keep_adjoint_state); otherwise they are recomputed.  The Perrin chi factor's
+ * temperature dependence enters dL/dT. */
 int marfa_pt_tables_adjoint(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
                             double *grad_lines, double *grad_levels);
 /* marfa_pt_tables_adjoint with the gradients in device memory (grad_lines [4][n], grad_levels
@@ -153,6 +155,14 @@ int marfa_pt_tables_adjoint(marfa_ctx *ctx, const marfa_job *job, const float *g
  * the GPU (the PyTorch operators in marfa_amd/csrc/torch_ops.cpp). */
 int marfa_pt_tables_adjoint_dev(marfa_ctx *ctx, const marfa_job *job, const float *grad_out, void *stream,
                                 double *grad_lines, double *grad_levels);
+
+/* on != 0: forwards of this context keep, beside the level-0 sums they always keep, the
+ * cascade's limiter choices for a following adjoint of the same job (3.2 KB per record, about
+ * 1 % of the forward's time), so that the adjoint does not recompute the finer grids; on == 0:
+ * they do not.  It switches on by itself once an adjoint has run on the context.  No
+ * counterpart in the reference (which has no adjoint); for differentiable callers
+ * (marfa_amd/autograd.py switches it on). */
+int marfa_keep_adjoint_state(marfa_ctx *ctx, int32_t on);
 
 /* Replace the per-line S, gamma_air, n_T and delta of the context's line list (n values each,
  * device pointers) by device-to-device copies on stream: the centres, and with them the

@@ -26,6 +26,7 @@ class PTTableModel:
         self.job = job
         self.device = torch.device("cuda", device)
         self.ctx = lbl.LBLContext(device)
+        self.ctx.keep_adjoint_state(True)  # forwards keep what the adjoint reuses
         self.ctx.set_tips(tips)
         self.spec = lbl.JobSpec.from_job(job, precision)
         self.ctx.set_lines(job.lines)

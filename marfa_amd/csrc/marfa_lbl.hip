@@ -932,6 +932,10 @@ struct marfa_ctx {
     // changes whenever lines, levels or TIPS do.
     Buf ckpt;
     uint64_t gen = 0;
+    // forwards keep the cascade's limiter choices too (marfa_keep_adjoint_state, or an
+    // adjoint has run on this context; the adjoint of a forward that did not keep them
+    // recomputes them)
+    bool adjoint_seen = false;
     struct CkptKey {
         bool valid = false;
         bool lim = false;  // the limiter choices beside the sums were written by a forward run
@@ -1685,10 +1689,11 @@ int marfa_pt_tables(marfa_ctx *c, const marfa_job *job, float *out_dev, float *o
     // phases are timed (and the host waits for them) unless the call is asynchronous
     double *ck = ckpt_for(c, w);
     if (w.nrec > 0 && !ck) return MARFA_E_HIP;
+    const bool keep_lim = ck && c->adjoint_seen;
     rc = run(c, *job, L, w, out, st, out_host != nullptr || !async, nullptr, nullptr, false, nullptr, ck, false,
-             ck ? ckpt_lim(c, w) : nullptr);
+             keep_lim ? ckpt_lim(c, w) : nullptr);
     if (rc) return rc;
-    ckpt_set(c, *job, st, w, true);
+    ckpt_set(c, *job, st, w, keep_lim);
     if (out_host) {
         HIPCHK(hipMemcpyAsync(out_host, out, total * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1728,10 +1733,11 @@ int marfa_pt_tables_f64(marfa_ctx *c, const marfa_job *job, double *out_dev, voi
     LevelSet L = level_set(c, *job);
     double *ck = ckpt_for(c, w);
     if (w.nrec > 0 && !ck) return MARFA_E_HIP;
+    const bool keep_lim = ck && c->adjoint_seen;
     rc = run(c, *job, L, w, nullptr, st, true, nullptr, out_dev, false, nullptr, ck, false,
-             ck ? ckpt_lim(c, w) : nullptr);
+             keep_lim ? ckpt_lim(c, w) : nullptr);
     if (rc) return rc;
-    ckpt_set(c, *job, st, w, true);
+    ckpt_set(c, *job, st, w, keep_lim);
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -1757,6 +1763,7 @@ static int adjoint_dev(marfa_ctx *c, const marfa_job *job, const float *grad_out
     // dL/dP_level and dL/dT_level: T also enters through pSelf = rho 10 k T / p0 (in the
     // line pass) -- the density is held fixed.  The level-0 sums come from the forward's
     // checkpoint when it holds this job with the current inputs, else they are recomputed.
+    c->adjoint_seen = true;
     const bool reuse = ckpt_holds(c, *job, st, w);
     const bool lim_ok = reuse && c->ck.lim;
     double *ck = reuse ? (double *)c->ckpt.p : ckpt_for(c, w);
@@ -1795,6 +1802,12 @@ int marfa_pt_tables_adjoint_dev(marfa_ctx *c, const marfa_job *job, const float 
     if (c->hP.empty()) return set_err(c, MARFA_E_STATE, "levels must be set");
     (void)hipSetDevice(c->device);
     return adjoint_dev(c, job, grad_out, stream ? (hipStream_t)stream : c->stream, grad_lines, grad_levels, false);
+}
+
+int marfa_keep_adjoint_state(marfa_ctx *c, int32_t on) {
+    if (!c) return MARFA_E_ARG;
+    c->adjoint_seen = on != 0;
+    return 0;
 }
 
 int marfa_update_line_params(marfa_ctx *c, const float *S, const float *gair, const float *nT, const float *delta,

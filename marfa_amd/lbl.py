@@ -75,6 +75,7 @@ def library() -> C.CDLL:
     L.marfa_pt_tables_noalloc.argtypes = [vp, C.POINTER(_Job), vp, vp]
     L.marfa_pt_tables_adjoint_dev.argtypes = [vp, C.POINTER(_Job), vp, vp, vp, vp]
     L.marfa_update_line_params.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.marfa_keep_adjoint_state.argtypes = [vp, C.c_int32]
     L.marfa_ctx_info.argtypes = [vp, i64p, i32p, i32p]
     L.marfa_batch_create.argtypes = [C.c_int, C.c_int32, C.c_double, C.POINTER(C.c_void_p)]
     L.marfa_batch_destroy.argtypes = [vp]
@@ -101,7 +102,7 @@ EXPORTS = ("marfa_ctx_create", "marfa_ctx_destroy", "marfa_last_error", "marfa_v
            "marfa_set_tips", "marfa_set_lines", "marfa_set_levels", "marfa_windows",
            "marfa_lbl_scheme", "marfa_pt_tables", "marfa_pt_tables_f64", "marfa_pt_tables_adjoint",
            "marfa_pt_tables_adjoint_dev", "marfa_update_line_params", "marfa_ctx_info", "marfa_count_samples",
-           "marfa_pt_tables_noalloc",
+           "marfa_pt_tables_noalloc", "marfa_keep_adjoint_state",
            "marfa_grid_sums", "marfa_last_timings", "marfa_batch_create", "marfa_batch_destroy",
            "marfa_batch_last_error", "marfa_batch_set_tips", "marfa_batch_add_job", "marfa_batch_tune",
            "marfa_batch_capture", "marfa_batch_launch", "marfa_batch_info", "marfa_batch_records",
@@ -278,6 +279,11 @@ class LBLContext:
         self._check(self.lib.marfa_pt_tables_f64(self.h, C.byref(j), C.c_void_p(out.data_ptr()),
                                                  C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         return out
+
+    def keep_adjoint_state(self, on: bool = True):
+        """Forwards keep the cascade's limiter choices for a following adjoint
+        (marfa_keep_adjoint_state)."""
+        self._check(self.lib.marfa_keep_adjoint_state(self.h, 1 if on else 0))
 
     def pt_tables_adjoint(self, spec: JobSpec, grad_out_ptr: int, stream: int = 0):
         """dL/d(S, gamma_air, n_T, delta) per line ([4][n], float64) and dL/d(T, P) per level

@@ -318,18 +318,20 @@ def test_finite_differences_config4():
 
 @pytest.mark.gpu
 def test_adjoint_checkpoint_reuse_and_invalidation():
-    """The adjoint reuses the forward's level-0 sums and cascade limiter choices only for the
-    inputs they were computed from: reused (no grid sums recomputed), recomputed after the
-    same parameters are set again, and after a forward at other parameters followed by a
+    """The adjoint reuses the forward's level-0 sums and cascade limiter choices (the model's
+    context keeps them: marfa_keep_adjoint_state) only for the inputs they were computed
+    from: recomputed without a checkpoint, reused (no grid sums recomputed), recomputed after
+    the same parameters are set again, and after a forward at other parameters followed by a
     return to these (no forward), the gradients are the same, bit for bit."""
     from marfa_amd.autograd import PTTableModel
     job = adjoint_job(n=1500, nlev=2)
     model = PTTableModel(job, TIPS)
     try:
         p = model.params()
-        rec = model.records()  # forward: the checkpoint holds these parameters
-        g = torch.randn(rec.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
-        a1 = model.adjoint(g)  # reuses it
+        g = torch.randn(model.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+        a0 = model.adjoint(g)  # no checkpoint: everything recomputed
+        model.records()        # forward: the checkpoint holds these parameters' sums and choices
+        a1 = model.adjoint(g)  # reuses both
         model.set_params(*p)   # same values, new generation: recomputed
         a2 = model.adjoint(g)
         model.set_params(*perturbed(p, np.random.default_rng(2)))
@@ -338,7 +340,7 @@ def test_adjoint_checkpoint_reuse_and_invalidation():
         a3 = model.adjoint(g)
     finally:
         model.close()
-    for a in (a2, a3):
+    for a in (a0, a2, a3):
         assert torch.equal(a1[0], a[0]) and torch.equal(a1[1], a[1])
 
 
@@ -378,3 +380,27 @@ def test_fused_loss_against_numpy(dtype):
     assert np.all(grad[~m] == 0.0)
     got1 = ops.load().log_loss(kt, kht, None, 1.0)
     assert abs(float(got1) - float(np.sum(d * d))) <= 1e-12 * float(np.sum(d * d))
+
+
+@pytest.mark.gpu
+def test_keep_adjoint_state_switch():
+    """marfa_keep_adjoint_state only decides what the forward keeps: the adjoint after a
+    forward that kept the limiter choices equals, bit for bit, the one after a forward that
+    did not (which recomputes the finer grids)."""
+    from marfa_amd import lbl
+    job = adjoint_job(n=1500, nlev=2, chi=2)
+    out = []
+    for keep in (False, True):
+        ctx = lbl.LBLContext(0)
+        try:
+            ctx.keep_adjoint_state(keep)
+            ctx.load_job(job, TIPS)
+            spec = lbl.JobSpec.from_job(job, lbl.FP32K)
+            lo, _, _, _ = ctx.windows(spec)
+            rec = torch.empty((len(job.atm), lo.shape[0], lbl.NT), dtype=torch.float32, device="cuda")
+            ctx.pt_tables_device(spec, rec.data_ptr(), 0)  # the context's stream, as the adjoint's
+            g = torch.randn(rec.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+            out.append(ctx.pt_tables_adjoint(spec, g.data_ptr()))
+        finally:
+            ctx.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
