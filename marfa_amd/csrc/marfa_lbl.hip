@@ -1268,7 +1268,6 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     // bucket index of shifted centres: 1/512 cm-1 buckets over every record's reach
     RunGeom G0;
     geom_of(job, W, G0);
-    const double bbase = G0.bbase;
     const int NB = G0.NB;
     const size_t shp_bytes = job.precision == MARFA_FP32K ? line_record_bytes<float>() : line_record_bytes<double>();
 
@@ -1314,7 +1313,7 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     }
     const RunArrays D = arrays_at(p_lvl, p_win, nlev, nrec);
     float *dP = D.P, *dT = D.T, *dmul = D.mul;
-    double *dps = D.pself, *dsc = D.sscale;
+    double *dps = D.pself;
     int64_t *dlo = D.lo, *dhi = D.hi;
     double *dorg = D.org;
     if (!capture) {
