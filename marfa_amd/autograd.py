@@ -48,7 +48,8 @@ class PTTableModel:
         f = lambda t: t.detach().to(self.device, torch.float32).contiguous()
         new = tuple(f(t) for t in (S, gair, nT, delta, T, P))
         ops.set_line_params(self.ctx, *new[:4])
-        if not (torch.equal(new[4], self._params[4]) and torch.equal(new[5], self._params[5])):
+        # one host synchronisation for both level comparisons
+        if torch.cat([new[4] != self._params[4], new[5] != self._params[5]]).any().item():
             ops.set_levels(self.ctx, new[5], new[4], self._rho)
         self._params = tuple(t.clone() for t in new)
         self.version += 1

@@ -932,6 +932,12 @@ struct marfa_ctx {
     // changes whenever lines, levels or TIPS do.
     Buf ckpt;
     uint64_t gen = 0;
+    // what run() last copied into lvl and win (host images, and the stream it copied on):
+    // the same values for the same stream are not copied again.  Cleared by any other
+    // writer of those buffers and when they are reallocated.
+    std::vector<char> up_lvl, up_win;
+    hipStream_t up_stream = nullptr;
+    bool up_valid = false;
     // forwards keep the cascade's limiter choices too (marfa_keep_adjoint_state, or an
     // adjoint has run on this context; the adjoint of a forward that did not keep them
     // recomputes them)
@@ -1306,10 +1312,13 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     // window arrays [nrec]: lo, hi, origin
     void *p_lvl = c->lvl.p, *p_win = c->win.p;
     if (!capture) {
+        // a reallocation (capacity change) may return the old address with other contents
+        const size_t lvl0 = c->lvl.bytes, win0 = c->win.bytes;
         p_lvl = get(c->lvl, lvl_bytes);
         if (!p_lvl) return MARFA_E_HIP;
         p_win = get(c->win, win_bytes);
         if (!p_win) return MARFA_E_HIP;
+        if (arena || c->lvl.bytes != lvl0 || c->win.bytes != win0) c->up_valid = false;
     }
     const RunArrays D = arrays_at(p_lvl, p_win, nlev, nrec);
     float *dP = D.P, *dT = D.T, *dmul = D.mul;
@@ -1317,15 +1326,34 @@ int run(marfa_ctx *c, const marfa_job &job, const LevelSet &L, const Windows &W,
     int64_t *dlo = D.lo, *dhi = D.hi;
     double *dorg = D.org;
     if (!capture) {
-        HIPCHK(hipMemcpyAsync(dP, L.P.data(), nlev * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dT, L.T.data(), nlev * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dmul, L.outmul.data(), nlev * 4, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
-    }
-    if (!capture) {
-        HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dhi, W.hi.data(), nrec * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dorg, W.org.data(), nrec * 8, hipMemcpyHostToDevice, st));
+        // host images of what the level and window arrays receive
+        std::vector<char> il((size_t)nlev * 20), iw((size_t)nrec * 24);
+        memcpy(il.data(), L.P.data(), nlev * 4);
+        memcpy(il.data() + nlev * 4, L.T.data(), nlev * 4);
+        memcpy(il.data() + nlev * 8, L.outmul.data(), nlev * 4);
+        memcpy(il.data() + nlev * 12, L.pself.data(), nlev * 8);
+        memcpy(iw.data(), W.lo.data(), nrec * 8);
+        memcpy(iw.data() + nrec * 8, W.hi.data(), nrec * 8);
+        memcpy(iw.data() + nrec * 16, W.org.data(), nrec * 8);
+        const bool same = c->up_valid && c->up_stream == st;
+        c->up_valid = false;  // until every copy below is enqueued
+        if (!(same && il == c->up_lvl)) {
+            HIPCHK(hipMemcpyAsync(dP, L.P.data(), nlev * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dT, L.T.data(), nlev * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dmul, L.outmul.data(), nlev * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dps, L.pself.data(), nlev * 8, hipMemcpyHostToDevice, st));
+        }
+        if (!(same && iw == c->up_win)) {
+            HIPCHK(hipMemcpyAsync(dlo, W.lo.data(), nrec * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dhi, W.hi.data(), nrec * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(dorg, W.org.data(), nrec * 8, hipMemcpyHostToDevice, st));
+        }
+        if (!arena) {
+            c->up_lvl = std::move(il);
+            c->up_win = std::move(iw);
+            c->up_stream = st;
+            c->up_valid = true;
+        }
     }
 
     const size_t arr = (size_t)lb * nl;
@@ -1896,6 +1924,7 @@ int marfa_count_samples(marfa_ctx *c, const marfa_job *job, int64_t *samples) {
         one.T = {L.T[l]};
         one.pself = {L.pself[l]};
         one.outmul = {1.f};
+        c->up_valid = false;  // the level and window buffers get other values here
         if (ensure(c, c->lvl, 64 + 3 * 4 + 2 * 8)) return MARFA_E_HIP;
         for (int q = 0; q < 4; q++)
             if (ensure(c, c->s->params[q], (size_t)nl * 8)) return MARFA_E_HIP;

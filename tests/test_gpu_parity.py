@@ -198,6 +198,32 @@ def test_determinism(ctx, tips):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_level_and_window_changes_between_runs(ctx, tips):
+    """A context keeps the level and window arrays of its last run on the device and copies
+    them again only when they change: runs whose levels, windows or both differ from the
+    previous run's (with the same array sizes) equal the same runs in fresh contexts."""
+    from marfa_amd import lbl
+    base = small_job(seed=23)
+    other_lev = small_job(seed=23, p0=50.0, p1=2e-3)
+    other_win = small_job(seed=23, a=4001.0, b=4031.0)
+    seq = [base, other_lev, other_win, base, base]
+    got = []
+    ctx.load_job(base, tips)
+    for job in seq:
+        ctx.set_levels(job.atm)
+        got.append(ctx.pt_tables(lbl.JobSpec.from_job(job, lbl.FP32K)))
+    for i, job in enumerate(seq[:3]):
+        fresh = lbl.LBLContext(0)
+        try:
+            fresh.load_job(job, tips)
+            ref = fresh.pt_tables(lbl.JobSpec.from_job(job, lbl.FP32K))
+        finally:
+            fresh.close()
+        assert np.array_equal(got[i].view(np.uint32), ref.view(np.uint32)), f"run {i}"
+    assert np.array_equal(got[3].view(np.uint32), got[0].view(np.uint32))
+    assert np.array_equal(got[4].view(np.uint32), got[0].view(np.uint32))
+
+
 @pytest.mark.parametrize("case", ["aligned", "offset", "start_below_cutoff", "wide_cut", "past_lines"])
 def test_windows_match_oracle(ctx, tips, case):
     """Subinterval windows and record numbers (app/main.f90:116-144, :196-248)."""
