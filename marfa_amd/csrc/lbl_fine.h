@@ -530,6 +530,11 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
             act = src.centre(p, u);  // staged lines always; sorted records: centre lines only
             w = side ? 10.0 - u : u;
         }
+        // the last position of a lane's walk on grid k, w - (n_k - 1) H_k, is the first of its
+        // walk on grid k + 1 (n_{k+1} >> 1 = n_k), formed by the same FP64 operations from the
+        // same w: its value is carried instead of evaluated again
+        R carry = R(0);
+        bool has_carry = false;
 #pragma unroll 1
         for (int k = LANES_K0; k <= 9; k++) {
             const double hq = 0.5 * hk(k);
@@ -545,8 +550,9 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
                     key = (side ? npair : 0) + pi;
                     const bool five = nn & 1;
                     const double q0 = (double)(4 * gi - 4);
+                    if (has_carry) a0 = carry;
 #pragma unroll 1
-                    for (int e = 0; e < (five ? 5 : 3); e++) {
+                    for (int e = has_carry ? 1 : 0; e < (five ? 5 : 3); e++) {
                         const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
                         if (e == 0) a0 = v;
                         else if (e == 1) a1 = v;
@@ -556,6 +562,10 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
                         } else if (e == 3) a3 = v;
                         else a4 = v;
                     }
+                    carry = five ? a4 : a2;
+                    has_carry = true;
+                } else {
+                    has_carry = false;
                 }
             }
             // slots P1 M1 L1 P2 M2 L2 of the lane's pair (right walk: the mirror image)

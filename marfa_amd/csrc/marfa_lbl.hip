@@ -277,6 +277,11 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
     const double u = nus - org;
     // classes as the deposit dispatch takes them (src/LBL.f90:81-98 on the shifted centre)
     const bool left = ok && nus < org, right = ok && nus > org + 10.0, centre = ok && !left && !right && u < 10.0;
+    // a centre line's last position on grid k of a walk, w - (n_k - 1) H_k, is the first of
+    // that walk on grid k + 1 (n_{k+1} >> 1 = n_k), formed by the same FP64 operations from
+    // the same w: its value is carried (per walk) instead of evaluated again
+    R carry0 = R(0), carry1 = R(0);
+    bool has0 = false, has1 = false;
 #pragma unroll 1
     for (int k = 1; k <= KNEAR; k++) {
         const double Hk = hk(k), hq = 0.5 * Hk;
@@ -310,8 +315,10 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
             }
             R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // e = 0..4 and e = 2 once more (n odd)
             if (key >= 0) {
+                const bool hc = centre && (side ? has1 : has0);
+                if (hc) v[0] = side ? carry1 : carry0;
 #pragma unroll 1
-                for (int e = 0; e < (five ? 5 : 3); e++) {
+                for (int e = hc ? 1 : 0; e < (five ? 5 : 3); e++) {
                     const double m = (C + (double)e) * hq;
                     const R x = shape_eval<R, true, CHI>(plus ? B + m : B - m, p, ctl);
                     if (e == 0) v[0] = x;
@@ -321,6 +328,16 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
                         if (five) v[5] = x;
                     } else if (e == 3) v[3] = x;
                     else v[4] = x;
+                }
+            }
+            if (centre) {
+                const R last = five ? v[4] : v[2];
+                if (side) {
+                    carry1 = last;
+                    has1 = key >= 0;
+                } else {
+                    carry0 = last;
+                    has0 = key >= 0;
                 }
             }
             // slots P1 M1 L1 P2 M2 L2 of the pair (right walk: the mirror image)
