@@ -339,7 +339,7 @@ __device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2]
         if (has) {
             const int mu = (int)floor(u * 512.0);
             key = mu - M0 + 1;  // 0 .. 130
-#pragma unroll 1
+#pragma unroll
             for (int q = 0; q < 12; q++) {
                 const int m = mu - 1 + (q >> 2);
                 if (m < M0 || m >= M0 + 128) continue;
@@ -348,10 +348,7 @@ __device__ inline void fine_lanes(const Src &src, int t, float *buf, double v[2]
                 const double vhi = (double)(((NT - i0) >> 2) + 2) * H9;
                 const double xi = (double)(i0 + e - 1) * HF;
                 if (u < xi ? !(10.0 - u < vhi) : !(u < uhi)) continue;
-                const R x = shape_eval<R, true>(u - xi, p, src.ctl);
-#pragma unroll
-                for (int r = 0; r < 12; r++)
-                    if (r == q) a[r] = x;
+                a[q] = shape_eval<R, true>(u - xi, p, src.ctl);
             }
         }
         key = scan_key(has, key);
@@ -551,16 +548,13 @@ __device__ inline void level_lanes(const Src &src, int t, double *g) {
                     const bool five = nn & 1;
                     const double q0 = (double)(4 * gi - 4);
                     if (has_carry) a0 = carry;
-#pragma unroll 1
-                    for (int e = has_carry ? 1 : 0; e < (five ? 5 : 3); e++) {
-                        const R v = shape_eval<R, true>(w - (q0 + (double)e) * hq, p, src.ctl);
-                        if (e == 0) a0 = v;
-                        else if (e == 1) a1 = v;
-                        else if (e == 2) {
-                            a2 = v;
-                            if (five) a2o = v;
-                        } else if (e == 3) a3 = v;
-                        else a4 = v;
+                    else a0 = shape_eval<R, true>(w - q0 * hq, p, src.ctl);
+                    a1 = shape_eval<R, true>(w - (q0 + 1.0) * hq, p, src.ctl);
+                    a2 = shape_eval<R, true>(w - (q0 + 2.0) * hq, p, src.ctl);
+                    if (five) {
+                        a2o = a2;
+                        a3 = shape_eval<R, true>(w - (q0 + 3.0) * hq, p, src.ctl);
+                        a4 = shape_eval<R, true>(w - (q0 + 4.0) * hq, p, src.ctl);
                     }
                     carry = five ? a4 : a2;
                     has_carry = true;

@@ -315,19 +315,19 @@ __device__ inline void near_grids_lanes(bool ok, double nus, double org, const L
             }
             R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};  // e = 0..4 and e = 2 once more (n odd)
             if (key >= 0) {
-                const bool hc = centre && (side ? has1 : has0);
-                if (hc) v[0] = side ? carry1 : carry0;
-#pragma unroll 1
-                for (int e = hc ? 1 : 0; e < (five ? 5 : 3); e++) {
-                    const double m = (C + (double)e) * hq;
-                    const R x = shape_eval<R, true, CHI>(plus ? B + m : B - m, p, ctl);
-                    if (e == 0) v[0] = x;
-                    else if (e == 1) v[1] = x;
-                    else if (e == 2) {
-                        v[2] = x;
-                        if (five) v[5] = x;
-                    } else if (e == 3) v[3] = x;
-                    else v[4] = x;
+                // position e of the walk: X = B -+ (C + e) H_k / 2
+                auto at = [&](double e) {
+                    const double m = (C + e) * hq;
+                    return shape_eval<R, true, CHI>(plus ? B + m : B - m, p, ctl);
+                };
+                if (centre && (side ? has1 : has0)) v[0] = side ? carry1 : carry0;
+                else v[0] = at(0.0);
+                v[1] = at(1.0);
+                v[2] = at(2.0);
+                if (five) {
+                    v[5] = v[2];
+                    v[3] = at(3.0);
+                    v[4] = at(4.0);
                 }
             }
             if (centre) {
