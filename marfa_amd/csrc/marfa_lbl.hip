@@ -558,17 +558,15 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     Far n1{}, n2{};
     if ((int)threadIdx.x < nfar) n1 = far_at(threadIdx.x);
     if ((int)threadIdx.x + BLOCK < nfar) n2 = far_at(threadIdx.x + BLOCK);
-    for (int v = threadIdx.x; v < nfar; v += BLOCK) {
+    // one far line v of the thread (its fields f, loaded two lines ahead)
+    auto far_line = [&](int v, const Far &f) __attribute__((always_inline)) {
         const int j = far_j(v);
-        const Far f = n1;
-        n1 = n2;
-        if (v + 2 * BLOCK < nfar) n2 = far_at(v + 2 * BLOCK);
-        if ((uint32_t)(f.idx - lo32) >= span) continue;  // the record's window (unshifted centres)
+        if ((uint32_t)(f.idx - lo32) >= span) return;  // the record's window (unshifted centres)
         const double u = f.nus - org;
         const bool lw = v < nleft;  // the lines before Ja lie left of the subinterval, those from Jb right
         // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
         const double dmin = lw ? -u : u - 10.0;
-        if (lw ? dmin > cut : dmin >= cut) continue;
+        if (lw ? dmin > cut : dmin >= cut) return;
         const R base = lw ? (R)dmin : (R)(dmin + 10.0);  // |X| at node 0
         const R step = lw ? R(0.5) : R(-0.5);
         // every node, every role: when even the nearest node, |X| >= dmin, is on the Lorentz
@@ -601,20 +599,20 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
             if (!one_law) {
                 vslow0 = min(vslow0, v);
                 vslow1 = v;
-                continue;
+                return;
             }
             if ((key != rkey && nrun > 0) || nrun == FAR_RUN) flush();
             rkey = key;
             nrun++;
             far_run(base, step, f.gl2, F, run);
-            continue;
+            return;
         }
         if (lor) {
             // the FP64 form: the Lorentz numerator (times chi) is one FP64 factor
             if (nochi) {
                 const double lorc = (double)f.lorc;
                 far_nodes<R>(base, step, f.gl2, [&](int q, R inv) { pos[q].add((double)inv * lorc); });
-                continue;
+                return;
             }
             if constexpr (kBands && !kRuns) {
                 const int b = chi_band<R, CHI>((R)dmin);
@@ -627,7 +625,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
                         if (q > 0) lchi *= rho;
                         pos[q].add((double)inv * lchi);
                     });
-                    continue;
+                    return;
                 }
             }
             LineShape<R> pl;
@@ -638,11 +636,20 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
                 const R ax = q == 0 ? base : fma(step, (R)q, base);
                 pos[q].add((double)chilor_abs<R, CHI>(ax, pl, ctl));
             }
-            continue;
+            return;
         }
         const LineShape<R> p = shp[j];
 #pragma unroll
         for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
+    };
+    // two lines per trip, so that each prefetched set of fields is consumed and reloaded in
+    // place (the lines keep their order: v, v + BLOCK, v + 2 BLOCK, ...)
+    for (int v = threadIdx.x; v < nfar; v += 2 * BLOCK) {
+        far_line(v, n1);
+        if (v + 2 * BLOCK < nfar) n1 = far_at(v + 2 * BLOCK);
+        if (v + BLOCK >= nfar) break;
+        far_line(v + BLOCK, n2);
+        if (v + 3 * BLOCK < nfar) n2 = far_at(v + 3 * BLOCK);
     }
 
     if (kRuns && nrun > 0) flush();
