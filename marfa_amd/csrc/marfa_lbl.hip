@@ -25,6 +25,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/marfa.h"
@@ -522,17 +523,17 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
     // FP32 form: the far records after the batch's line records
     const FarRec *farr = reinterpret_cast<const FarRec *>(reinterpret_cast<const LineShape<R> *>(a.shp) +
                                                           (int64_t)a.nlev * a.nl) + (int64_t)lev * a.nl;
-    auto far_at = [&](int v) {
+    auto far_rec = [&](int j) {  // the fields of sorted line j
         Far f;
         if constexpr (kRuns) {
-            const FarRec q = farr[far_j(v)];
+            const FarRec q = farr[j];
             f.nus = q.nus;
             f.idx = q.idx;
             f.xs = q.xs;
             f.lorc = q.lorc;
             f.gl2 = q.gl2;
         } else {
-            const LineShape<R> &q = shp[far_j(v)];
+            const LineShape<R> &q = shp[j];
             f.nus = q.nus;
             f.idx = q.idx;
             f.xs = q.xs;
@@ -541,6 +542,7 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         }
         return f;
     };
+    auto far_at = [&](int v) { return far_rec(far_j(v)); };
     // FP32 form: does chi keep one band b over the 21 nodes?  |X| is monotone in q, so the
     // bands of nodes 0 and 20 (as the nodes are formed) decide; Pollack: band 2 from 10 cm-1
     // on, and its other bands are narrower than the 10 cm-1 the nodes span
@@ -555,15 +557,13 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
         }
     };
     int vslow0 = 0x7fffffff, vslow1 = -1;  // the FP32 form's lines for the second pass
-    Far n1{}, n2{};
-    if ((int)threadIdx.x < nfar) n1 = far_at(threadIdx.x);
-    if ((int)threadIdx.x + BLOCK < nfar) n2 = far_at(threadIdx.x + BLOCK);
-    // one far line v of the thread (its fields f, loaded two lines ahead)
-    auto far_line = [&](int v, const Far &f) __attribute__((always_inline)) {
-        const int j = far_j(v);
+    // one far line v of the thread (its fields f, loaded two lines ahead); LW: the lines
+    // before Ja (v < nleft) lie left of the subinterval, those from Jb right
+    auto far_line = [&](int v, const Far &f, auto LWc) __attribute__((always_inline)) {
+        constexpr bool lw = decltype(LWc)::value;
+        const int j = lw ? J0 + v : Jb + (v - nleft);
         if ((uint32_t)(f.idx - lo32) >= span) return;  // the record's window (unshifted centres)
         const double u = f.nus - org;
-        const bool lw = v < nleft;  // the lines before Ja lie left of the subinterval, those from Jb right
         // leftLBL / rightLBL cutoff tests (src/LineGridCalc.f90:23, :508-509)
         const double dmin = lw ? -u : u - 10.0;
         if (lw ? dmin > cut : dmin >= cut) return;
@@ -642,14 +642,29 @@ __device__ __forceinline__ void coarse_item(const KArgs &a, int r) {
 #pragma unroll
         for (int q = 0; q <= 20; q++) pos[q].add((double)shape_eval<R, false, CHI>(u - 0.5 * q, p, ctl));
     };
-    // two lines per trip, so that each prefetched set of fields is consumed and reloaded in
-    // place (the lines keep their order: v, v + BLOCK, v + 2 BLOCK, ...)
-    for (int v = threadIdx.x; v < nfar; v += 2 * BLOCK) {
-        far_line(v, n1);
-        if (v + 2 * BLOCK < nfar) n1 = far_at(v + 2 * BLOCK);
-        if (v + BLOCK >= nfar) break;
-        far_line(v + BLOCK, n2);
-        if (v + 3 * BLOCK < nfar) n2 = far_at(v + 3 * BLOCK);
+    // the thread's lines v = tid, tid + BLOCK, ... of [vbeg, vend) on one side, two per trip, so
+    // that each prefetched set of fields is consumed and reloaded in place
+    auto far_range = [&](int vbeg, int vend, auto LWc) __attribute__((always_inline)) {
+        constexpr bool lw = decltype(LWc)::value;
+        auto rec = [&](int v) { return far_rec(lw ? J0 + v : Jb + (v - nleft)); };
+        Far n1{}, n2{};
+        if (vbeg < vend) n1 = rec(vbeg);
+        if (vbeg + BLOCK < vend) n2 = rec(vbeg + BLOCK);
+        for (int v = vbeg; v < vend; v += 2 * BLOCK) {
+            far_line(v, n1, LWc);
+            if (v + 2 * BLOCK < vend) n1 = rec(v + 2 * BLOCK);
+            if (v + BLOCK >= vend) break;
+            far_line(v + BLOCK, n2, LWc);
+            if (v + 3 * BLOCK < vend) n2 = rec(v + 3 * BLOCK);
+        }
+    };
+    // the same lines in the same order as one pass over [0, nfar): the left range, then the
+    // right range from the thread's first v >= nleft
+    {
+        const int tid = (int)threadIdx.x;
+        far_range(tid, nleft, std::true_type{});
+        const int vr = tid >= nleft ? tid : tid + (nleft - tid + BLOCK - 1) / BLOCK * BLOCK;
+        far_range(vr, nfar, std::false_type{});
     }
 
     if (kRuns && nrun > 0) flush();
