@@ -61,30 +61,45 @@ struct LineAdj {
     const ShapeCtl *ctl;
     double u;         // nu* - origin
     double aS, aX, aD, aL, aT;
-    __device__ inline void acc(double x, float gs) {
-        if (gs == 0.f) return;
+    __device__ inline void acc(double x, double gs) {
+        if (gs == 0.0) return;
         const SampleGrad<R> s = shape_grad<R>(u - x, *p, *ctl);
-        aS += (double)gs * (double)s.v;
-        aX += (double)gs * (double)s.dX;
-        aD += (double)gs * (double)s.dgD;
-        aL += (double)gs * (double)s.dgL;
-        aT += (double)gs * (double)s.dT;
+        aS += gs * (double)s.v;
+        aX += gs * (double)s.dX;
+        aD += gs * (double)s.dgD;
+        aL += gs * (double)s.dgL;
+        aT += gs * (double)s.dT;
     }
     // positions q = qlo..qhi (steps of H_k/2) of a run of cells of grid k: even q is the L
-    // edge of cell q/2 (q > qlo) and the P edge of cell q/2+1 (q < qhi); odd q a mid point
-    __device__ inline void walk(int k, int qlo, int qhi) {
+    // edge of cell q/2 (q > qlo) and the P edge of cell q/2+1 (q < qhi); odd q a mid point.
+    // A centre line's consecutive grids share one position (the last of grid k's left walk
+    // is the first of grid k+1's, the first of grid k's right walk the last of grid k+1's,
+    // x formed exactly either way), sampled once: cfirst / clast are node adjoints that
+    // another walk hands over for the first / last position, and with dfirst / dlast that
+    // position is not sampled here but its adjoint returned for the other walk.
+    __device__ inline double walk(int k, int qlo, int qhi, double cfirst = 0.0, double clast = 0.0,
+                                  bool dfirst = false, bool dlast = false) {
         const double h = 0.5 * hk(k);
+        double ret = 0.0;
         for (int q = qlo; q <= qhi; q++) {
-            float gs;
+            float g;
             if (q & 1) {
-                gs = gr[node_m(k, (q + 1) >> 1)];
+                g = gr[node_m(k, (q + 1) >> 1)];
             } else {
-                gs = 0.f;
-                if (q > qlo) gs += gr[node_l(k, q >> 1)];
-                if (q < qhi) gs += gr[node_p(k, (q >> 1) + 1)];
+                g = 0.f;
+                if (q > qlo) g += gr[node_l(k, q >> 1)];
+                if (q < qhi) g += gr[node_p(k, (q >> 1) + 1)];
+            }
+            double gs = g;
+            if (q == qlo) gs += cfirst;
+            if (q == qhi) gs += clast;
+            if ((q == qlo && dfirst) || (q == qhi && dlast)) {
+                ret = gs;
+                continue;
             }
             acc((double)q * h, gs);
         }
+        return ret;
     }
     // Does a wing line dmin >= 1 cm-1 outside the subinterval have its nearest node -- and
     // so every node -- on the chi-corrected Lorentz branch (the level-0 kernel's test)?
@@ -174,23 +189,34 @@ struct LineAdj {
     }
     // centerLBL (src/LineGridCalc.f90:181-494): RK(1), the two walks and the core points
     __device__ inline void centre() {
-        acc(0.0, gr[node_rk(1)]);
-        int ep = 0;
+        // walk ends e_k on the left (w = u) and right (w = 10 - u) sides
+        auto ends = [&](double w, int k) { return w >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(w, k)) - 1 : 0; };
+        // RK(1) is at x = 0, the first position of the grid-0 left walk when there is one
+        int ep = 0, e = ends(u, 0);
+        double c = 0.0;
+        if (e > 0) c = gr[node_rk(1)];
+        else acc(0.0, gr[node_rk(1)]);
         for (int k = 0; k <= 9; k++) {
-            const int e = u >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(u, k)) - 1 : 0;
-            if (e > 2 * ep) walk(k, 4 * ep, 2 * e);
+            const int en = k < 9 ? ends(u, k + 1) : 0;
+            // grid k + 1 walks (en > 2 e) from this walk's last position
+            if (e > 2 * ep) c = walk(k, 4 * ep, 2 * e, c, 0.0, false, k < 9 && en > 2 * e);
             ep = e;
+            e = en;
         }
         const int fu = (int)floor(u / HF);
         if (u >= HF)
             for (int i = 4 * ep + 2; i <= fu + 1; i++) acc((double)(i - 1) * HF, gr[node_rk(i)]);
         const double v = 10.0 - u;
         int er = 0;
+        e = ends(v, 0);
+        c = 0.0;
         for (int k = 0; k <= 9; k++) {
-            const int e = v >= 2.0 * hk(k) ? (int)floor(__builtin_ldexp(v, k)) - 1 : 0;
+            const int en = k < 9 ? ends(v, k + 1) : 0;
             const int two_nt = 20 << k;
-            if (e > 2 * er) walk(k, two_nt - 2 * e, two_nt - 4 * er);
+            // grid k + 1 walks (en > 2 e) up to this walk's first position
+            if (e > 2 * er) c = walk(k, two_nt - 2 * e, two_nt - 4 * er, 0.0, c, k < 9 && en > 2 * e, false);
             er = e;
+            e = en;
         }
         for (int i = fu + 2; i <= NT - 4 * er; i++) acc((double)(i - 1) * HF, gr[node_rk(i)]);
     }
