@@ -491,20 +491,6 @@ __device__ inline SampleGrad<R> shape_grad(double X64, const LineShape<R> &p, co
     return o;
 }
 
-// Key of a lane for a key-segmented lane scan.  Lanes with a value (has) keep their key;
-// a lane without one takes the key of the nearest lane before it that has one (its own
-// values are zero), or -1 before the first.  Along the lanes with values the keys are
-// non-decreasing, but lanes without one can sit between two lanes of one key (a line
-// outside the record's window among the sorted candidates); inheriting the key keeps every
-// key's lanes contiguous, so the scan's segment ends hold each key's sum exactly once.
-__device__ inline int scan_key(bool has, int key) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t before = __ballot(has) & ((1ull << lane) - 1ull);
-    const int from = before ? 63 - __clzll(before) : lane;
-    const int k = __shfl(key, from);
-    return has ? key : (before ? k : -1);
-}
-
 // ---------------------------------------------------------------- lane scans by DPP
 // Lane moves through the VALU's data-parallel primitives (no LDS round trip, unlike a
 // shuffle): dpp_move returns in each lane the value of the lane that CTRL selects, or `old`
@@ -557,6 +543,26 @@ __device__ inline void seg_scan(int key, T (&v)[N]) {
 
 // the next lane's key (-3 in lane 63): a lane ends its key's run when this differs
 __device__ inline int next_key(int key) { return dpp_move<0x130, 0xF>(-3, key); }
+
+// Key of a lane for a key-segmented lane scan.  Lanes with a value (has) keep their key;
+// a lane without one takes the key of the nearest lane before it that has one (its own
+// values are zero), or -1 before the first.  Along the lanes with values the keys are
+// non-decreasing, but lanes without one can sit between two lanes of one key (a line
+// outside the record's window among the sorted candidates); inheriting the key keeps every
+// key's lanes contiguous, so the scan's segment ends hold each key's sum exactly once.
+// Formed as an inclusive max-scan of (has ? key : -1) across the lanes (as the keys with
+// values do not decrease, the maximum up to a lane is the key of the last such lane up to
+// it): DPP steps, no LDS round trip.  Every lane of the wavefront must be active.
+__device__ inline int scan_key(bool has, int key) {
+    int k = has ? key : -1;
+    k = max(k, dpp_move<0x111, 0xF>(-1, k));
+    k = max(k, dpp_move<0x112, 0xF>(-1, k));
+    k = max(k, dpp_move<0x114, 0xF>(-1, k));
+    k = max(k, dpp_move<0x118, 0xF>(-1, k));
+    k = max(k, dpp_move<0x142, 0xA>(-1, k));
+    k = max(k, dpp_move<0x143, 0xC>(-1, k));
+    return k;
+}
 
 // ---------------------------------------------------------------- accumulators
 // Node and record sums are FP64 in both precision forms: an FP32 sample enters an FP64 sum
